@@ -1,0 +1,164 @@
+"""Benchmark of the wind-tunnel hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload f1|cube|large]
+                    [--storage f32|f16s|bf16s] [--no-cpu]
+
+Workload (default "f1"): the generic F1 body, 320x96x96 half tunnel at 4.5 cm
+cells, FP32 populations, D3Q19 regularized + Smagorinsky, moving floor, sponge
+outlet, forces split front/rear/rest, heat scalar from six seeded sources.  One
+step is one fused stream-collide pass over the whole tunnel; the heat scalar runs
+at the reference's cadence (2 heat steps per 8 stream-collide steps,
+engine.ts:708-712).  Inputs are voxelized on the GPU before the timed region.
+
+Prints one JSON line with MLUPS (cells x steps / s), the HBM roofline of the
+stream-collide kernel (its average launch time from HIP events recorded on its
+own stream during the timed region) and the CPU baseline (the OpenMP build of
+the FP32 restatement of the reference kernels, timed on a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+for _p in (ROOT, ROOT / "f1-wind-tunnel_amd"):
+    if str(_p) not in sys.path:
+        sys.path.insert(0, str(_p))
+
+METRIC = "MLUPS & % HBM roofline; 25-pt aero-map wall time, surrogate/batched vs LBM"
+HBM_PEAK_GBS = 8000.0
+
+
+def bytes_per_update(storage: str) -> int:
+    # read 19 populations + write 19 populations + 1 flag byte (fields not counted:
+    # they are written only on the last step of each call)
+    s = 4 if storage == "f32" else 2
+    return 2 * 19 * s + 1
+
+
+def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
+    """FP32 restatement of the reference kernels (OpenMP C), timed on a bounded
+    sample of the same workload."""
+    import numpy as np
+
+    from oracle import oracle as orc
+
+    ref = orc.LBMOracle("f32_omp")
+    sdf = orc.SDFOracle()
+    g = orc.grid(wl.dims, wl.h, wl.origin)
+    sim = orc.OSim(wl.u_lb, wl.tau0, wl.smag, 0.0)
+    solid, emit, _ = sdf.voxelize(g, wl.shape, prims=wl.prims)
+    f, fields = ref.init(g, sim)
+    t0 = time.perf_counter()
+    f, fields, _ = ref.run(g, sim, f, solid, fields, emit, 2)
+    dt = time.perf_counter() - t0
+    steps = max(2, min(400, int(budget_s / max(dt / 2, 1e-6))))
+    t0 = time.perf_counter()
+    ref.run(g, sim, f, solid, fields, emit, steps)
+    dt = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(wl.cells * steps / dt / 1e6, 2), "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"{steps} FP32 stream-collide steps of the {wl.name} workload "
+                      f"({wl.dims[0]}x{wl.dims[1]}x{wl.dims[2]}), CPU restatement of the reference "
+                      f"kernels (gcc -O3 -fopenmp), {dt:.1f} s"}
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10_000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="f1", choices=["f1", "cube", "large"])
+    ap.add_argument("--storage", default="f32", choices=["f32", "f16s", "bf16s"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the tunnel would shard as x-slabs with one halo exchange per step; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import aerolbm
+    from aerolbm import workloads as W
+
+    wl = {"f1": W.f1, "cube": W.cube, "large": W.large}[args.workload]()
+    print(f"[bench] workload {wl.name} {wl.dims} storage {args.storage}: voxelize + init", file=sys.stderr, flush=True)
+    e = aerolbm.HipAeroEngine(storage=args.storage, **wl.engine_kwargs())
+    e.set_shape(wl.shape, prims=wl.prims)
+    e.reset()
+
+    frame_steps = 8
+    heat_dt = frame_steps / 2
+
+    def run(nsteps: int) -> None:
+        done = 0
+        while done < nsteps:
+            k = min(frame_steps, nsteps - done)
+            e.step(k)
+            if k == frame_steps:
+                e.heat_step(2, dt=heat_dt)
+            done += k
+
+    run(args.warmup)
+    e.sync()
+    print(f"[bench] warmup {args.warmup} steps done; timing {args.steps} steps", file=sys.stderr, flush=True)
+    e.profile_begin(max_calls=(args.steps + frame_steps - 1) // frame_steps + 1)
+    e.sync()
+    t0 = time.perf_counter()
+    run(args.steps)
+    e.sync()
+    wall = time.perf_counter() - t0
+    lbm_ms, lbm_steps = e.profile_end()
+    d, q, nsteps = e.read_forces_raw(reset=True)
+    guard = e.guard_count()
+
+    mlups = wl.cells * args.steps / wall / 1e6
+    bpu = bytes_per_update(args.storage)
+    launch_ms = lbm_ms / max(lbm_steps, 1)
+    achieved = wl.cells * bpu / (launch_ms / 1e3) / 1e9
+    result = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.storage == "f32" else f"{args.storage[:-1]} storage, f32 math",
+        "data": "synthetic: seeded analytic SDF geometry voxelized on the GPU",
+        "config": {
+            "workload": f"{wl.name}: {wl.dims[0]}x{wl.dims[1]}x{wl.dims[2]} half tunnel, h={wl.h:.4g} m, "
+                        f"D3Q19 regularized+Smagorinsky Cs={wl.smag}, u={wl.u_lb}, tau0={wl.tau0}, "
+                        f"{args.storage} populations, heat 2 per 8 steps",
+            "cells": wl.cells,
+            "storage": args.storage,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": f"k_step<{args.storage}>",
+            "bytes_per_cell": bpu,
+            "launch_us": round(launch_ms * 1e3, 2),
+        },
+        "forces": {"steps": nsteps, "rest_fx": d[0], "front_fy": d[4], "rear_fy": d[7], "guard_hits": guard},
+    }
+    if not args.no_cpu:
+        print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
+        result["cpu_baseline"] = cpu_baseline(wl)
+    print(json.dumps(result))
+    e.close()
+
+
+if __name__ == "__main__":
+    main()

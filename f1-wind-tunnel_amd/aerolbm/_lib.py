@@ -1,0 +1,130 @@
+"""ctypes binding of libaero_lbm.so (the C ABI in include/aero_lbm.h).
+
+The library is the product: if it is missing or fails to load, every entry point
+raises; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+PKG_ROOT = Path(__file__).resolve().parent.parent
+LIB_PATH = Path(os.environ.get("AERO_LBM_LIB", PKG_ROOT / "lib" / "libaero_lbm.so"))
+
+AERO_OK = 0
+ERRORS = {-1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "bad state"}
+
+STORE_F32, STORE_F16S, STORE_BF16S = 0, 1, 2
+STORAGE_NAMES = {"f32": STORE_F32, "f16s": STORE_F16S, "bf16s": STORE_BF16S}
+
+
+class Grid(C.Structure):
+    """== WGSL struct Grid (src/engine/wgsl.ts:6)."""
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("n", C.c_uint32),
+                ("ox", C.c_float), ("oy", C.c_float), ("oz", C.c_float), ("h", C.c_float)]
+
+
+class Sim(C.Structure):
+    """== WGSL struct Sim (src/engine/wgsl.ts:142)."""
+    _fields_ = [("u_in", C.c_float), ("tau0", C.c_float), ("smag", C.c_float), ("pad", C.c_float)]
+
+
+class Heat(C.Structure):
+    """== WGSL struct Heat (src/engine/wgsl.ts:461)."""
+    _fields_ = [("dt", C.c_float), ("cool", C.c_float), ("diffuse", C.c_float), ("pad", C.c_float)]
+
+
+class Shape(C.Structure):
+    """== WGSL struct Shape (src/engine/wgsl.ts:10-17), packed like engine.ts:429-448."""
+    _fields_ = [("kind", C.c_uint32), ("profile_mode", C.c_uint32), ("pw", C.c_uint32), ("ph", C.c_uint32),
+                ("front_flap", C.c_float), ("rear_main", C.c_float), ("rear_flap", C.c_float), ("ride_h", C.c_float),
+                ("aoa", C.c_float), ("width", C.c_float), ("axis_y", C.c_float), ("p_scale", C.c_float),
+                ("p_origin", C.c_float * 2), ("heat_brakes", C.c_float), ("heat_engine", C.c_float),
+                ("heat_rad", C.c_float), ("front_wing", C.c_float), ("rear_wing", C.c_float), ("sidepod", C.c_float),
+                ("cover", C.c_float), ("nose", C.c_float), ("rear_ride", C.c_float), ("duct", C.c_float)]
+
+
+class Prim(C.Structure):
+    """aero_prim: one primitive of a kind-16 scene."""
+    _fields_ = [("type", C.c_uint32), ("material", C.c_uint32), ("frame", C.c_uint32), ("pad", C.c_uint32),
+                ("a", C.c_float * 12)]
+
+
+class Rig(C.Structure):
+    """aero_rig == WGSL struct Rig (src/engine/car-model.ts:60-69)."""
+    _fields_ = [("front", (C.c_float * 4) * 71), ("hinges", C.c_float * 4), ("angles", C.c_float * 4),
+                ("extra", C.c_float * 4), ("fly", C.c_float * 4), ("grids", (C.c_float * 4) * 12)]
+
+
+assert C.sizeof(Grid) == 32 and C.sizeof(Sim) == 16 and C.sizeof(Heat) == 16
+assert C.sizeof(Shape) == 96 and C.sizeof(Prim) == 64
+
+_P = C.c_void_p
+_U32 = C.c_uint32
+_PROTOS = {
+    "aero_version": (_U32, []),
+    "aero_global_error": (C.c_char_p, []),
+    "aero_last_error": (C.c_char_p, [_P]),
+    "aero_create": (C.c_int, [C.POINTER(Grid), C.POINTER(Sim), C.c_int, C.POINTER(_P)]),
+    "aero_destroy": (C.c_int, [_P]),
+    "aero_set_sim": (C.c_int, [_P, C.POINTER(Sim)]),
+    "aero_init": (C.c_int, [_P]),
+    "aero_voxelize": (C.c_int, [_P, C.POINTER(Shape), _P, _P, _U32]),
+    "aero_set_car_model": (C.c_int, [_P, _P, _P, _P, C.POINTER(Rig)]),
+    "aero_set_flags": (C.c_int, [_P, _P, _P]),
+    "aero_step": (C.c_int, [_P, _U32]),
+    "aero_step_timed": (C.c_int, [_P, _U32, C.POINTER(C.c_float)]),
+    "aero_heat_step": (C.c_int, [_P, C.POINTER(Heat), _U32]),
+    "aero_read_forces": (C.c_int, [_P, _P, _P, C.POINTER(C.c_uint64), C.c_int]),
+    "aero_read_fields": (C.c_int, [_P, _P]),
+    "aero_read_flags": (C.c_int, [_P, _P, _P]),
+    "aero_read_temps": (C.c_int, [_P, _P]),
+    "aero_write_temps": (C.c_int, [_P, _P]),
+    "aero_read_populations": (C.c_int, [_P, _P]),
+    "aero_write_populations": (C.c_int, [_P, _P]),
+    "aero_guard_count": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
+    "aero_sync": (C.c_int, [_P]),
+    "aero_profile_begin": (C.c_int, [_P, _U32]),
+    "aero_profile_end": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
+}
+
+_lib = None
+
+
+class AeroError(RuntimeError):
+    pass
+
+
+def load() -> C.CDLL:
+    """Loads the HIP library; raises if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise AeroError(f"{LIB_PATH} not built: run python f1-wind-tunnel_amd/build.py")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_PROTOS)
+
+
+def check(rc: int, ctx=None, what: str = "") -> None:
+    if rc == AERO_OK:
+        return
+    lib = load()
+    msg = (lib.aero_last_error(ctx) if ctx else lib.aero_global_error()) or b""
+    raise AeroError(f"{what}: {ERRORS.get(rc, rc)}: {msg.decode(errors='replace')}")
+
+
+def ptr(a: np.ndarray) -> C.c_void_p:
+    assert a.flags["C_CONTIGUOUS"]
+    return C.c_void_p(a.ctypes.data)

@@ -1,0 +1,319 @@
+"""Host-side mirror of the simulation surface of the reference's AeroEngine.
+
+Reference: src/engine/engine.ts:159-769 (constructor :226-281, reset :371-381,
+setShape :426-470, frame :574-729 (compute part :708-715), maybeReadForces
+:739-761, debugRead :763-768).  The GPU work goes through libaero_lbm.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+from typing import Callable, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import Grid, Heat, Prim, Shape, Sim, check, ptr
+
+# src/engine/engine.ts:14-28,110,249,263
+QUALITIES = {
+    "low": (168, 50, 38),
+    "medium": (232, 68, 52),
+    "high": (296, 88, 66),
+}
+DOMAIN_LENGTH = 10.4
+ORIGIN_X = -2.4
+FORCE_SCALE = 10_000
+AIR_DENSITY = 1.225
+PROFILE_W, PROFILE_H = 256, 128
+LATTICE_U = 0.0925
+TAU0 = 0.508
+SMAG = 0.17
+HEAT_COOL = 0.004
+HEAT_DIFFUSE = 0.04
+DEG = math.pi / 180
+COOLING_HEAT = [1.55, 1, 0.6]
+COOLING_DUCT = [0, 1, 1.3]
+
+
+class ShapeKind:
+    """src/engine/engine.ts:30 (+ the library's primitive scene)."""
+    F1, Sphere, Cube, Wing, Custom, W17, Empty = 0, 1, 2, 3, 4, 5, 6
+    Scene = 16
+
+
+@dataclass
+class ShapeParams:
+    """src/engine/engine.ts:36-65 (simulation-relevant fields)."""
+    kind: int = ShapeKind.F1
+    frontWing: int = 0
+    rearWing: int = 0
+    sidepod: int = 0
+    cover: int = 0
+    nose: int = 0
+    rearRideH: float = 0.12
+    rearMainDeg: float = 0.0
+    frontFlapDeg: float = 0.0
+    rearFlapDeg: float = 0.0
+    rideH: float = 0.10
+    aoaDeg: float = 6.0
+    customMode: int = 0
+    customWidth: float = 0.5
+    customAxisY: float = 1.0
+    heatBrakes: bool = True
+    heatEngine: bool = True
+    heatRad: bool = True
+    cooling: int = 1
+
+
+@dataclass
+class CustomProfile:
+    """src/engine/engine.ts:67-73."""
+    sdf: np.ndarray
+    originX: float
+    originY: float
+    cell: float
+
+
+@dataclass
+class ForcePair:
+    drag: float
+    down: float
+
+
+@dataclass
+class Forces:
+    """src/engine/engine.ts:100-107."""
+    dragN: float
+    downforceN: float
+    body: ForcePair
+    frontWing: ForcePair
+    rearWing: ForcePair
+
+
+def shape_uniform(p: ShapeParams, profile: Optional[CustomProfile] = None) -> Shape:
+    """Packs the Shape uniform exactly as engine.ts:429-448 does."""
+    s = Shape()
+    s.kind = int(p.kind)
+    s.profile_mode = int(p.customMode)
+    s.pw, s.ph = PROFILE_W, PROFILE_H
+    s.front_flap = (p.frontFlapDeg + 14) * DEG
+    s.rear_main = 9 * DEG
+    s.rear_flap = (p.rearFlapDeg + 32) * DEG
+    s.ride_h = p.rideH
+    s.aoa = p.aoaDeg * DEG
+    s.width = p.customWidth
+    s.axis_y = p.customAxisY
+    s.p_scale = profile.cell if profile else 0.02
+    s.p_origin[0] = profile.originX if profile else 0.0
+    s.p_origin[1] = profile.originY if profile else 0.0
+    cool = p.cooling if p.cooling is not None else 1
+    s.heat_brakes = 520 if p.heatBrakes else 0
+    s.heat_engine = 620 * COOLING_HEAT[cool] if p.heatEngine else 0
+    s.heat_rad = 75 * COOLING_HEAT[cool] if p.heatRad else 0
+    s.front_wing, s.rear_wing, s.sidepod = p.frontWing, p.rearWing, p.sidepod
+    s.cover, s.nose, s.rear_ride = p.cover, p.nose, p.rearRideH
+    s.duct = COOLING_DUCT[cool]
+    return s
+
+
+class HipAeroEngine:
+    """Drop-in for the simulation half of AeroEngine, on one MI355X.
+
+    ``quality`` selects the reference presets (h = 10.4 m / nx, origin (-2.4, 0, 0));
+    explicit ``dims``/``h``/``origin`` describe any other tunnel (the benchmark
+    workloads need h = 1/22 m or 4.5 cm cells, which the preset rule cannot give).
+    """
+
+    def __init__(self, quality: str = "medium", *, dims: Optional[Sequence[int]] = None,
+                 h: Optional[float] = None, origin: Optional[Sequence[float]] = None,
+                 u_lb: float = LATTICE_U, tau0: float = TAU0, smag: float = SMAG,
+                 storage: str = "f32", kmh: float = 250.0):
+        nx, ny, nz = dims if dims is not None else QUALITIES[quality]
+        self.dims = (int(nx), int(ny), int(nz))
+        self.n = nx * ny * nz
+        self.h = float(h) if h is not None else DOMAIN_LENGTH / nx
+        self.origin = tuple(origin) if origin is not None else (ORIGIN_X, 0.0, 0.0)
+        self.u_lb = float(u_lb)
+        self.tau0 = float(tau0)
+        self.smag = float(smag)
+        self.kmh = float(kmh)
+        self.storage = storage
+        self.heat_params = Heat(3.0, HEAT_COOL, HEAT_DIFFUSE, 0.0)
+        self.on_forces: Optional[Callable[[Forces], None]] = None
+        self.frame_no = 0
+        self.steps_since_read = 0
+        self._lib = _lib.load()
+        self.grid = Grid(nx, ny, nz, self.n, self.origin[0], self.origin[1], self.origin[2], self.h)
+        self.sim = Sim(self.u_lb, self.tau0, self.smag, 0.0)
+        ctx = C.c_void_p()
+        check(self._lib.aero_create(C.byref(self.grid), C.byref(self.sim), _lib.STORAGE_NAMES[storage],
+                                    C.byref(ctx)), None, "aero_create")
+        self.ctx = ctx
+        self.profile: Optional[CustomProfile] = None
+
+    # -- lifetime --
+    def close(self) -> None:
+        if getattr(self, "ctx", None):
+            check(self._lib.aero_destroy(self.ctx), None, "aero_destroy")
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _c(self, rc: int, what: str) -> None:
+        check(rc, self.ctx, what)
+
+    # -- engine surface --
+    def set_sim(self, u_lb: Optional[float] = None, tau0: Optional[float] = None, smag: Optional[float] = None):
+        if u_lb is not None:
+            self.u_lb = float(u_lb)
+        if tau0 is not None:
+            self.tau0 = float(tau0)
+        if smag is not None:
+            self.smag = float(smag)
+        self.sim = Sim(self.u_lb, self.tau0, self.smag, 0.0)
+        self._c(self._lib.aero_set_sim(self.ctx, C.byref(self.sim)), "aero_set_sim")
+
+    def reset(self) -> None:
+        """engine.ts:371-381: equilibrium fill of both buffers, temperatures zeroed."""
+        self._c(self._lib.aero_init(self.ctx), "aero_init")
+        self.steps_since_read = 0
+
+    def set_profile(self, profile: CustomProfile) -> None:
+        self.profile = profile
+
+    def set_shape(self, p, prims: Optional[Sequence[Prim]] = None) -> None:
+        """engine.ts:426-470: packs the Shape uniform and voxelizes."""
+        shape = p if isinstance(p, Shape) else shape_uniform(p, self.profile)
+        prof = None
+        if shape.kind not in (ShapeKind.Sphere, ShapeKind.Cube, ShapeKind.Wing, ShapeKind.Empty,
+                              ShapeKind.Scene, ShapeKind.F1, ShapeKind.W17):
+            src = self.profile.sdf if self.profile else np.full(PROFILE_W * PROFILE_H, 10.0, np.float32)
+            prof = np.ascontiguousarray(src, dtype=np.float32)
+        parr = None
+        if prims:
+            parr = (Prim * len(prims))(*prims)
+        self._c(self._lib.aero_voxelize(self.ctx, C.byref(shape), ptr(prof) if prof is not None else None,
+                                        C.cast(parr, C.c_void_p) if parr is not None else None,
+                                        len(prims) if prims else 0), "aero_voxelize")
+
+    def set_flags(self, solid: np.ndarray, emit: np.ndarray) -> None:
+        s = np.ascontiguousarray(solid, dtype=np.uint32).reshape(-1)
+        e = np.ascontiguousarray(emit, dtype=np.float32).reshape(-1)
+        assert s.size == self.n and e.size == self.n
+        self._c(self._lib.aero_set_flags(self.ctx, ptr(s), ptr(e)), "aero_set_flags")
+
+    def step(self, nsteps: int = 1) -> None:
+        self._c(self._lib.aero_step(self.ctx, int(nsteps)), "aero_step")
+        self.steps_since_read += int(nsteps)
+
+    def step_timed(self, nsteps: int) -> float:
+        ms = C.c_float()
+        self._c(self._lib.aero_step_timed(self.ctx, int(nsteps), C.byref(ms)), "aero_step_timed")
+        self.steps_since_read += int(nsteps)
+        return float(ms.value)
+
+    def profile_begin(self, max_calls: int = 4096) -> None:
+        self._c(self._lib.aero_profile_begin(self.ctx, int(max_calls)), "aero_profile_begin")
+
+    def profile_end(self):
+        """Returns (GPU ms of the timed aero_step calls, steps they covered)."""
+        ms = C.c_float()
+        n = C.c_uint64()
+        self._c(self._lib.aero_profile_end(self.ctx, C.byref(ms), C.byref(n)), "aero_profile_end")
+        return float(ms.value), int(n.value)
+
+    def heat_step(self, nsteps: int = 1, dt: Optional[float] = None) -> None:
+        hp = Heat(self.heat_params.dt if dt is None else float(dt), self.heat_params.cool,
+                  self.heat_params.diffuse, 0.0)
+        self._c(self._lib.aero_heat_step(self.ctx, C.byref(hp), int(nsteps)), "aero_heat_step")
+
+    def frame(self, steps: int = 8) -> None:
+        """The compute part of engine.ts:574-729: evenSteps LBM steps, then two heat
+        steps with dt = evenSteps/2; forces are read back every 8th frame."""
+        self.frame_no += 1
+        even = max(2, steps - steps % 2)
+        self.step(even)
+        self.heat_step(2, dt=even / 2)
+        if self.frame_no % 8 == 0:
+            f = self.read_forces(reset=True)
+            if f is not None and self.on_forces:
+                self.on_forces(f)
+
+    def read_forces_raw(self, reset: bool = False):
+        """Returns (fp64 sums[9], fixed-point sums[9], steps)."""
+        d = np.zeros(9, np.float64)
+        q = np.zeros(9, np.int64)
+        steps = C.c_uint64()
+        self._c(self._lib.aero_read_forces(self.ctx, ptr(d), ptr(q), C.byref(steps), int(reset)), "aero_read_forces")
+        if reset:
+            self.steps_since_read = 0
+        return d, q, int(steps.value)
+
+    def force_scale(self, steps: int) -> float:
+        """engine.ts:752 without the 1/FORCE_SCALE factor: lattice force per step -> N."""
+        v_real = self.kmh / 3.6
+        return (1.0 / steps) * AIR_DENSITY * (v_real / self.u_lb) ** 2 * self.h * self.h * 2
+
+    def read_forces(self, reset: bool = True, quantized: bool = False) -> Optional[Forces]:
+        """engine.ts:739-761: {dragN, downforceN, body, frontWing, rearWing}."""
+        d, q, steps = self.read_forces_raw(reset)
+        if steps == 0:
+            return None
+        raw = q.astype(np.float64) / FORCE_SCALE if quantized else d
+        s = self.force_scale(steps)
+        pair = lambda g: ForcePair(raw[g * 3] * s, -raw[g * 3 + 1] * s)
+        body, fw, rw = pair(0), pair(1), pair(2)
+        return Forces(body.drag + fw.drag + rw.drag, body.down + fw.down + rw.down, body, fw, rw)
+
+    def guard_count(self) -> int:
+        c = C.c_uint64()
+        self._c(self._lib.aero_guard_count(self.ctx, C.byref(c)), "aero_guard_count")
+        return int(c.value)
+
+    def debug_read(self) -> dict:
+        """engine.ts:763-768."""
+        return {"fields": self.fields(), "solid": self.flags()[0], "temps": self.temps()}
+
+    def fields(self) -> np.ndarray:
+        out = np.empty((self.n, 4), np.float32)
+        self._c(self._lib.aero_read_fields(self.ctx, ptr(out)), "aero_read_fields")
+        return out
+
+    def flags(self):
+        s = np.empty(self.n, np.uint32)
+        e = np.empty(self.n, np.float32)
+        self._c(self._lib.aero_read_flags(self.ctx, ptr(s), ptr(e)), "aero_read_flags")
+        return s, e
+
+    def temps(self) -> np.ndarray:
+        out = np.empty(self.n, np.float32)
+        self._c(self._lib.aero_read_temps(self.ctx, ptr(out)), "aero_read_temps")
+        return out
+
+    def write_temps(self, t: np.ndarray) -> None:
+        a = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        self._c(self._lib.aero_write_temps(self.ctx, ptr(a)), "aero_write_temps")
+
+    def populations(self) -> np.ndarray:
+        out = np.empty((19, self.n), np.float32)
+        self._c(self._lib.aero_read_populations(self.ctx, ptr(out)), "aero_read_populations")
+        return out
+
+    def write_populations(self, f: np.ndarray) -> None:
+        a = np.ascontiguousarray(f, dtype=np.float32).reshape(19, self.n)
+        self._c(self._lib.aero_write_populations(self.ctx, ptr(a)), "aero_write_populations")
+
+    def sync(self) -> None:
+        self._c(self._lib.aero_sync(self.ctx), "aero_sync")

@@ -1,0 +1,120 @@
+// Internal declarations of libaero_lbm.so (not part of the ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/aero_lbm.h"
+
+namespace aero {
+
+// D3Q19 velocity set in the reference's order (src/engine/wgsl.ts:124-133).  The
+// arrays are constexpr so that every per-direction branch below folds at compile time.
+constexpr int kCX[19] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int kCY[19] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int kCZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+constexpr int kOPP[19] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+constexpr int kZMIR[19] = {0, 1, 2, 3, 4, 6, 5, 7, 8, 9, 10, 13, 14, 11, 12, 17, 18, 15, 16};
+// f32 weights: the WGSL literals 0.33333334, 0.055555556, 0.027777778.
+constexpr float kW0 = (float)(1.0 / 3.0);
+constexpr float kW1 = (float)(1.0 / 18.0);
+constexpr float kW2 = (float)(1.0 / 36.0);
+__host__ __device__ constexpr float wt(int i) { return i == 0 ? kW0 : (i <= 6 ? kW1 : kW2); }
+
+// Per-cell flag byte, derived from solid/emit after every voxelize or set_flags.
+enum : uint8_t {
+    FL_GROUP = 0x03,  // own solid group: 0 fluid, 1 rest, 2 front wing, 3 rear wing
+    FL_NEAR = 0x04,   // some pull source of this fluid cell is solid (needs neighbour groups)
+    FL_DUCT = 0x08,   // emit < 0: W17 duct cell (no forces, duct relaxation)
+    FL_FLOOR0 = 0x10, // y == 0 floor cell touching a solid: moving-floor speed 0
+    FL_HOTNB = 0x20,  // some in-range 6-neighbour has emit > 0 (heat source relaxation)
+};
+
+// Work tile of the stream-collide kernel: 64 cells along x (one wavefront) by 4
+// rows along y (four wavefronts) in one z plane.
+constexpr int kTileX = 64;
+constexpr int kTileY = 4;
+constexpr int kBlock = kTileX * kTileY;
+
+// Force/guard partial sums owned by one tile.  Each tile adds its own contribution
+// in place every step, so the totals are reproducible run to run; aero_read_forces
+// reduces the slots in a fixed order.
+struct alignas(16) TileSlot {
+    double d[9];        // FP64 sums of the per-cell force, slot g*3+k
+    long long q[9];     // fixed-point sums round(f*1e4) (x and y only)
+    unsigned long long guard;
+    unsigned long long pad;
+};
+
+struct StepArgs {
+    const void* fin;
+    void* fout;
+    const uint8_t* flags;
+    float4* fields;         // written when write_fields != 0
+    float* prev_u;          // outlet plane (x = nx-1) velocities, 3 floats per (y, z)
+    TileSlot* slots;
+    const float* tau_x;     // tau0 + outlet/inlet sponge per x (wgsl.ts:259-260)
+    uint64_t ns;            // population plane stride (elements)
+    uint32_t nx, ny, nz;
+    uint32_t tiles_x, tiles_y, ntiles;
+    float u_in;
+    float smag_c;           // f32((18*1.41421356)) * Cs * Cs  (wgsl.ts:261)
+    float feq_in[19];       // feq(i, 1, (u_in, 0, 0))
+    float floor_term[19];   // 6 * w_i * c_x,i * u_in  (moving floor, wgsl.ts:210)
+    int write_fields;
+};
+
+}  // namespace aero
+
+struct aero_ctx {
+    aero_grid grid;
+    aero_sim sim;
+    int storage;
+    int device;
+    hipStream_t stream;
+    hipEvent_t ev0, ev1;
+    uint64_t ns;            // plane stride: n rounded up to 256
+    size_t pop_bytes;       // bytes of one population buffer
+    void* f[2];
+    int cur;                // buffer holding the newest populations
+    uint32_t* solid;
+    float* emit;
+    uint8_t* flags;
+    float4* fields;
+    float* prev_u;
+    float* temps[2];
+    int tcur;
+    float* tau_x;
+    aero::TileSlot* slots;
+    uint32_t tiles_x, tiles_y, ntiles;
+    void* red_out;          // device reduction result (TileSlot)
+    uint64_t steps;         // steps since the last force reset
+    // W17 car model (kind 5)
+    uint16_t* car_body;
+    uint16_t* car_parts;
+    uint32_t parts_dims[3];
+    aero_rig rig;
+    int has_car;
+    // Live timing of stream-collide bursts (aero_profile_begin/end): one event pair
+    // per aero_step call, recorded on the context stream.
+    std::vector<hipEvent_t> prof_ev;
+    size_t prof_used;
+    uint64_t prof_steps;
+    int prof_on;
+    std::string err;
+};
+
+namespace aero {
+// Kernel launchers (lbm.hip, heat.hip, voxelize.hip).
+hipError_t launch_init(aero_ctx* c);
+hipError_t launch_flags(aero_ctx* c);
+hipError_t launch_step(aero_ctx* c, int write_fields);
+hipError_t launch_reduce_slots(aero_ctx* c);
+hipError_t launch_heat(aero_ctx* c, const aero_heat* h);
+hipError_t launch_voxelize(aero_ctx* c, const aero_shape* s, const float* d_profile,
+                           const aero_prim* d_prims, uint32_t nprims, const float* host_extra);
+hipError_t launch_pack_pops(aero_ctx* c, const float* d_in);    // f32 SoA -> storage
+hipError_t launch_unpack_pops(aero_ctx* c, float* d_out);       // storage -> f32 SoA
+void fill_step_consts(const aero_ctx* c, StepArgs* a);
+}  // namespace aero

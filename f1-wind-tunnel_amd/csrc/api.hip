@@ -1,0 +1,437 @@
+// C ABI of libaero_lbm.so (declared in include/aero_lbm.h).
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "aero_internal.h"
+
+using namespace aero;
+
+namespace {
+
+std::mutex g_err_mu;
+std::string g_err;
+
+void set_global_error(const std::string& m) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    g_err = m;
+}
+
+int fail(aero_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    else set_global_error(m);
+    return code;
+}
+
+int hip_fail(aero_ctx* c, hipError_t e, const char* where) {
+    std::string m = std::string(where) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")";
+    return fail(c, e == hipErrorOutOfMemory ? AERO_E_NOMEM : AERO_E_HIP, m);
+}
+
+#define HIPCHK(c, expr)                                           \
+    do {                                                          \
+        hipError_t e_ = (expr);                                   \
+        if (e_ != hipSuccess) return hip_fail((c), e_, #expr);    \
+    } while (0)
+
+#define CTXCHK(c)                                                                  \
+    do {                                                                           \
+        if (!(c)) return fail(nullptr, AERO_E_INVALID, "null context");           \
+        hipError_t e_ = hipSetDevice((c)->device);                                 \
+        if (e_ != hipSuccess) return hip_fail((c), e_, "hipSetDevice");            \
+    } while (0)
+
+// WGSL smoothstep in f32 (host side of the per-x relaxation-time table).
+float smoothstep_h(float e0, float e1, float x) {
+    float t = (x - e0) / (e1 - e0);
+    t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+    return t * t * (3.0f - 2.0f * t);
+}
+
+int upload_tau(aero_ctx* c) {
+    // tau0 + 0.4*smoothstep(NX-18, NX-1, x) + 0.25*(1 - smoothstep(0, 10, x))   (wgsl.ts:259-260)
+    const uint32_t nx = c->grid.nx;
+    std::vector<float> t(nx);
+    const float fnx = (float)nx;
+    for (uint32_t x = 0; x < nx; x++) {
+        const float fx = (float)x;
+        t[x] = c->sim.tau0 + 0.4f * smoothstep_h(fnx - 18.0f, fnx - 1.0f, fx) +
+               0.25f * (1.0f - smoothstep_h(0.0f, 10.0f, fx));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->tau_x, t.data(), nx * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AERO_OK;
+}
+
+void free_ctx(aero_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    void* ptrs[] = {c->f[0], c->f[1], c->solid, c->emit, c->flags, c->fields, c->prev_u, c->temps[0],
+                    c->temps[1], c->tau_x, c->slots, c->red_out, c->car_body, c->car_parts};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    for (hipEvent_t ev : c->prof_ev) (void)hipEventDestroy(ev);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+size_t store_bytes(int storage) { return storage == AERO_STORE_F32 ? 4 : 2; }
+
+}  // namespace
+
+extern "C" {
+
+uint32_t aero_version(void) { return 1u; }
+
+const char* aero_global_error(void) {
+    std::lock_guard<std::mutex> lk(g_err_mu);
+    return g_err.c_str();
+}
+
+const char* aero_last_error(const aero_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ctx** out) {
+    if (!grid || !sim || !out) return fail(nullptr, AERO_E_INVALID, "aero_create: null argument");
+    *out = nullptr;
+    if (grid->nx < 4 || grid->ny < 2 || grid->nz < 1)
+        return fail(nullptr, AERO_E_INVALID, "aero_create: grid must be at least 4x2x1");
+    const uint64_t n = (uint64_t)grid->nx * grid->ny * grid->nz;
+    if (n != grid->n) return fail(nullptr, AERO_E_INVALID, "aero_create: grid.n != nx*ny*nz");
+    if (n > (1ull << 31)) return fail(nullptr, AERO_E_INVALID, "aero_create: more than 2^31 cells");
+    if (grid->nz > 65535u) return fail(nullptr, AERO_E_INVALID, "aero_create: nz above 65535");
+    if (storage != AERO_STORE_F32 && storage != AERO_STORE_F16S && storage != AERO_STORE_BF16S)
+        return fail(nullptr, AERO_E_INVALID, "aero_create: unknown storage format");
+    if (!(grid->h > 0.0f) || !isfinite(grid->h)) return fail(nullptr, AERO_E_INVALID, "aero_create: bad cell size");
+    if (!(sim->tau0 > 0.5f)) return fail(nullptr, AERO_E_INVALID, "aero_create: tau0 must exceed 0.5");
+
+    aero_ctx* c = new (std::nothrow) aero_ctx();
+    if (!c) return fail(nullptr, AERO_E_NOMEM, "aero_create: host allocation failed");
+    c->grid = *grid;
+    c->sim = *sim;
+    c->storage = storage;
+    hipError_t e = hipGetDevice(&c->device);
+    if (e != hipSuccess) { free_ctx(c); set_global_error(std::string("hipGetDevice: ") + hipGetErrorString(e)); return AERO_E_HIP; }
+    c->ns = (n + 255) & ~(uint64_t)255;
+    c->pop_bytes = (size_t)19 * c->ns * store_bytes(storage);
+    c->tiles_x = (grid->nx + kTileX - 1) / kTileX;
+    c->tiles_y = (grid->ny + kTileY - 1) / kTileY;
+    c->ntiles = c->tiles_x * c->tiles_y * grid->nz;
+    const size_t outlet = (size_t)grid->ny * grid->nz * 3;
+
+    struct Alloc { void** p; size_t bytes; } allocs[] = {
+        {&c->f[0], c->pop_bytes},
+        {&c->f[1], c->pop_bytes},
+        {(void**)&c->solid, n * 4},
+        {(void**)&c->emit, n * 4},
+        {(void**)&c->flags, c->ns},
+        {(void**)&c->fields, n * 16},
+        {(void**)&c->prev_u, outlet * 4},
+        {(void**)&c->temps[0], n * 4},
+        {(void**)&c->temps[1], n * 4},
+        {(void**)&c->tau_x, (size_t)grid->nx * 4},
+        {(void**)&c->slots, (size_t)c->ntiles * sizeof(TileSlot)},
+        {&c->red_out, sizeof(TileSlot)},
+    };
+    for (auto& a : allocs) {
+        e = hipMalloc(a.p, a.bytes);
+        if (e != hipSuccess) {
+            std::string m = std::string("aero_create: hipMalloc(") + std::to_string(a.bytes) + "): " + hipGetErrorString(e);
+            free_ctx(c);
+            set_global_error(m);
+            return e == hipErrorOutOfMemory ? AERO_E_NOMEM : AERO_E_HIP;
+        }
+    }
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&c->ev0)) != hipSuccess || (e = hipEventCreate(&c->ev1)) != hipSuccess) {
+        free_ctx(c);
+        set_global_error(std::string("aero_create: stream/event: ") + hipGetErrorString(e));
+        return AERO_E_HIP;
+    }
+    // Empty tunnel until the first voxelize / set_flags.
+    e = hipMemsetAsync(c->solid, 0, n * 4, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->emit, 0, n * 4, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->flags, 0, c->ns, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * sizeof(TileSlot), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->temps[0], 0, n * 4, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->temps[1], 0, n * 4, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        free_ctx(c);
+        set_global_error(std::string("aero_create: clear: ") + hipGetErrorString(e));
+        return AERO_E_HIP;
+    }
+    if (upload_tau(c) != AERO_OK) {
+        set_global_error(c->err);
+        free_ctx(c);
+        return AERO_E_HIP;
+    }
+    *out = c;
+    return AERO_OK;
+}
+
+int aero_destroy(aero_ctx* c) {
+    if (!c) return fail(nullptr, AERO_E_INVALID, "aero_destroy: null context");
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    free_ctx(c);
+    return AERO_OK;
+}
+
+int aero_set_sim(aero_ctx* c, const aero_sim* sim) {
+    CTXCHK(c);
+    if (!sim) return fail(c, AERO_E_INVALID, "aero_set_sim: null sim");
+    if (!(sim->tau0 > 0.5f)) return fail(c, AERO_E_INVALID, "aero_set_sim: tau0 must exceed 0.5");
+    c->sim = *sim;
+    return upload_tau(c);
+}
+
+int aero_init(aero_ctx* c) {
+    CTXCHK(c);
+    HIPCHK(c, launch_init(c));
+    const uint64_t n = c->grid.n;
+    HIPCHK(c, hipMemsetAsync(c->temps[0], 0, n * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->temps[1], 0, n * 4, c->stream));
+    HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * sizeof(TileSlot), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->cur = 0;
+    c->tcur = 0;
+    c->steps = 0;
+    return AERO_OK;
+}
+
+int aero_voxelize(aero_ctx* c, const aero_shape* s, const float* profile, const aero_prim* prims, uint32_t nprims) {
+    CTXCHK(c);
+    if (!s) return fail(c, AERO_E_INVALID, "aero_voxelize: null shape");
+    const uint32_t kind = s->kind;
+    if (kind == AERO_KIND_F1 || kind == AERO_KIND_W17)
+        return fail(c, AERO_E_INVALID, "aero_voxelize: shape kind not available in this build");
+    const bool custom = !(kind == AERO_KIND_SPHERE || kind == AERO_KIND_CUBE || kind == AERO_KIND_WING ||
+                          kind == AERO_KIND_EMPTY || kind == AERO_KIND_SCENE);
+    if (custom) {
+        if (!profile) return fail(c, AERO_E_INVALID, "aero_voxelize: custom kind needs a profile");
+        if (s->pw == 0 || s->ph == 0 || s->pw > 4096 || s->ph > 4096)
+            return fail(c, AERO_E_INVALID, "aero_voxelize: profile size out of range");
+        if (!(s->p_scale > 0.0f)) return fail(c, AERO_E_INVALID, "aero_voxelize: p_scale must be positive");
+    }
+    if (kind == AERO_KIND_SCENE && (nprims == 0 || !prims || nprims > 4096))
+        return fail(c, AERO_E_INVALID, "aero_voxelize: scene needs 1..4096 primitives");
+
+    // Shape-only angles, evaluated once on the host (f1.ts:37, wgsl.ts:68,106).
+    float extra[4];
+    {
+        const float th = atanf((s->rear_ride - s->ride_h) / (4.4f - 1.0f));
+        extra[0] = cosf(th);
+        extra[1] = sinf(th);
+        extra[2] = cosf(-s->aoa);
+        extra[3] = sinf(-s->aoa);
+    }
+    float* d_profile = nullptr;
+    aero_prim* d_prims = nullptr;
+    int rc = AERO_OK;
+    hipError_t e = hipSuccess;
+    if (custom) {
+        const size_t bytes = (size_t)s->pw * s->ph * 4;
+        e = hipMalloc(&d_profile, bytes);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_profile, profile, bytes, hipMemcpyHostToDevice, c->stream);
+    }
+    if (e == hipSuccess && kind == AERO_KIND_SCENE) {
+        std::vector<aero_prim> hp(prims, prims + nprims);
+        for (auto& q : hp)
+            if (q.type == 3u) { q.a[10] = cosf(q.a[3]); q.a[11] = sinf(q.a[3]); }
+        e = hipMalloc(&d_prims, nprims * sizeof(aero_prim));
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(d_prims, hp.data(), nprims * sizeof(aero_prim), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    }
+    if (e == hipSuccess) e = launch_voxelize(c, s, d_profile, d_prims, nprims, extra);
+    if (e == hipSuccess) e = launch_flags(c);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) rc = hip_fail(c, e, "aero_voxelize");
+    if (d_profile) (void)hipFree(d_profile);
+    if (d_prims) (void)hipFree(d_prims);
+    return rc;
+}
+
+int aero_set_car_model(aero_ctx* c, const uint16_t*, const uint16_t*, const uint32_t*, const aero_rig*) {
+    CTXCHK(c);
+    return fail(c, AERO_E_INVALID, "aero_set_car_model: W17 model not available in this build");
+}
+
+int aero_set_flags(aero_ctx* c, const uint32_t* solid, const float* emit) {
+    CTXCHK(c);
+    if (!solid || !emit) return fail(c, AERO_E_INVALID, "aero_set_flags: null argument");
+    const uint64_t n = c->grid.n;
+    for (uint64_t i = 0; i < n; i++)
+        if (solid[i] > 3u) return fail(c, AERO_E_INVALID, "aero_set_flags: solid value above 3");
+    HIPCHK(c, hipMemcpyAsync(c->solid, solid, n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->emit, emit, n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_flags(c));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AERO_OK;
+}
+
+int aero_step(aero_ctx* c, uint32_t nsteps) {
+    CTXCHK(c);
+    const bool timed = c->prof_on && nsteps > 0 && c->prof_used + 2 <= c->prof_ev.size();
+    if (timed) HIPCHK(c, hipEventRecord(c->prof_ev[c->prof_used], c->stream));
+    for (uint32_t k = 0; k < nsteps; k++) HIPCHK(c, launch_step(c, k + 1 == nsteps));
+    if (timed) {
+        HIPCHK(c, hipEventRecord(c->prof_ev[c->prof_used + 1], c->stream));
+        c->prof_used += 2;
+        c->prof_steps += nsteps;
+    }
+    c->steps += nsteps;
+    return AERO_OK;
+}
+
+int aero_profile_begin(aero_ctx* c, uint32_t max_calls) {
+    CTXCHK(c);
+    if (max_calls == 0 || max_calls > 65536) return fail(c, AERO_E_INVALID, "aero_profile_begin: max_calls out of range");
+    for (hipEvent_t ev : c->prof_ev) (void)hipEventDestroy(ev);
+    c->prof_ev.clear();
+    for (uint32_t k = 0; k < 2 * max_calls; k++) {
+        hipEvent_t ev;
+        HIPCHK(c, hipEventCreate(&ev));
+        c->prof_ev.push_back(ev);
+    }
+    c->prof_used = 0;
+    c->prof_steps = 0;
+    c->prof_on = 1;
+    return AERO_OK;
+}
+
+int aero_profile_end(aero_ctx* c, float* ms, uint64_t* steps) {
+    CTXCHK(c);
+    if (!ms || !steps) return fail(c, AERO_E_INVALID, "aero_profile_end: null output");
+    if (!c->prof_on) return fail(c, AERO_E_STATE, "aero_profile_end: profiling not started");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double total = 0.0;
+    for (size_t k = 0; k + 1 < c->prof_used; k += 2) {
+        float t = 0.0f;
+        HIPCHK(c, hipEventElapsedTime(&t, c->prof_ev[k], c->prof_ev[k + 1]));
+        total += t;
+    }
+    *ms = (float)total;
+    *steps = c->prof_steps;
+    c->prof_on = 0;
+    return AERO_OK;
+}
+
+int aero_step_timed(aero_ctx* c, uint32_t nsteps, float* ms) {
+    CTXCHK(c);
+    if (!ms) return fail(c, AERO_E_INVALID, "aero_step_timed: null ms");
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    for (uint32_t k = 0; k < nsteps; k++) HIPCHK(c, launch_step(c, k + 1 == nsteps));
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    HIPCHK(c, hipEventSynchronize(c->ev1));
+    HIPCHK(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+    c->steps += nsteps;
+    return AERO_OK;
+}
+
+int aero_heat_step(aero_ctx* c, const aero_heat* h, uint32_t nsteps) {
+    CTXCHK(c);
+    if (!h) return fail(c, AERO_E_INVALID, "aero_heat_step: null heat");
+    for (uint32_t k = 0; k < nsteps; k++) HIPCHK(c, launch_heat(c, h));
+    return AERO_OK;
+}
+
+int aero_read_forces(aero_ctx* c, double sum_fp64[9], int64_t sum_q[9], uint64_t* steps, int reset) {
+    CTXCHK(c);
+    HIPCHK(c, launch_reduce_slots(c));
+    TileSlot r;
+    HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(TileSlot), hipMemcpyDeviceToHost, c->stream));
+    if (reset) HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * sizeof(TileSlot), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (sum_fp64) for (int k = 0; k < 9; k++) sum_fp64[k] = r.d[k];
+    if (sum_q) for (int k = 0; k < 9; k++) sum_q[k] = (int64_t)r.q[k];
+    if (steps) *steps = c->steps;
+    if (reset) c->steps = 0;
+    return AERO_OK;
+}
+
+int aero_guard_count(aero_ctx* c, uint64_t* count) {
+    CTXCHK(c);
+    if (!count) return fail(c, AERO_E_INVALID, "aero_guard_count: null count");
+    HIPCHK(c, launch_reduce_slots(c));
+    TileSlot r;
+    HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(TileSlot), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *count = r.guard;
+    return AERO_OK;
+}
+
+int aero_read_fields(aero_ctx* c, float* out) {
+    CTXCHK(c);
+    if (!out) return fail(c, AERO_E_INVALID, "aero_read_fields: null output");
+    HIPCHK(c, hipMemcpyAsync(out, c->fields, c->grid.n * 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AERO_OK;
+}
+
+int aero_read_flags(aero_ctx* c, uint32_t* solid, float* emit) {
+    CTXCHK(c);
+    if (!solid && !emit) return fail(c, AERO_E_INVALID, "aero_read_flags: null outputs");
+    if (solid) HIPCHK(c, hipMemcpyAsync(solid, c->solid, c->grid.n * 4, hipMemcpyDeviceToHost, c->stream));
+    if (emit) HIPCHK(c, hipMemcpyAsync(emit, c->emit, c->grid.n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AERO_OK;
+}
+
+int aero_read_temps(aero_ctx* c, float* out) {
+    CTXCHK(c);
+    if (!out) return fail(c, AERO_E_INVALID, "aero_read_temps: null output");
+    HIPCHK(c, hipMemcpyAsync(out, c->temps[c->tcur], c->grid.n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AERO_OK;
+}
+
+int aero_write_temps(aero_ctx* c, const float* in) {
+    CTXCHK(c);
+    if (!in) return fail(c, AERO_E_INVALID, "aero_write_temps: null input");
+    HIPCHK(c, hipMemcpyAsync(c->temps[c->tcur], in, c->grid.n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AERO_OK;
+}
+
+int aero_read_populations(aero_ctx* c, float* out) {
+    CTXCHK(c);
+    if (!out) return fail(c, AERO_E_INVALID, "aero_read_populations: null output");
+    const size_t bytes = (size_t)19 * c->grid.n * 4;
+    float* d = nullptr;
+    HIPCHK(c, hipMalloc(&d, bytes));
+    hipError_t e = launch_unpack_pops(c, d);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(c, e, "aero_read_populations");
+    return AERO_OK;
+}
+
+int aero_write_populations(aero_ctx* c, const float* in) {
+    CTXCHK(c);
+    if (!in) return fail(c, AERO_E_INVALID, "aero_write_populations: null input");
+    const size_t bytes = (size_t)19 * c->grid.n * 4;
+    float* d = nullptr;
+    HIPCHK(c, hipMalloc(&d, bytes));
+    hipError_t e = hipMemcpyAsync(d, in, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_pack_pops(c, d);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    if (e != hipSuccess) return hip_fail(c, e, "aero_write_populations");
+    return AERO_OK;
+}
+
+int aero_sync(aero_ctx* c) {
+    CTXCHK(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AERO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,89 @@
+// Semi-Lagrangian heat scalar for gfx950 (reference behaviour: HEAT,
+// src/engine/wgsl.ts:460-511).  One thread per cell: back-trace along the cell's
+// velocity, trilinear gather of the previous temperature (z < 0 mirrored, x < 0
+// cold), 6-neighbour smoothing, source relaxation and cooling.  The source test
+// reads neighbour emit values only where the flag byte marks a hot neighbour.
+#include "aero_internal.h"
+
+namespace aero {
+
+struct HeatArgs {
+    const float4* fields;
+    const float* emit;
+    const uint8_t* flags;
+    const float* tin;
+    float* tout;
+    uint32_t nx, ny, nz;
+    uint32_t pad;
+    float dt, cool, diffuse;
+};
+
+__device__ __forceinline__ uint64_t heat_idx(const HeatArgs& h, int x, int y, int z) {
+    if (z < 0) z = -z - 1;
+    x = x < 0 ? 0 : (x > (int)h.nx - 1 ? (int)h.nx - 1 : x);
+    y = y < 0 ? 0 : (y > (int)h.ny - 1 ? (int)h.ny - 1 : y);
+    z = z < 0 ? 0 : (z > (int)h.nz - 1 ? (int)h.nz - 1 : z);
+    return (uint64_t)x + (uint64_t)h.nx * ((uint64_t)y + (uint64_t)h.ny * (uint64_t)z);
+}
+__device__ __forceinline__ float heat_at(const HeatArgs& h, int x, int y, int z) {
+    if (x < 0) return 0.0f;
+    return h.tin[heat_idx(h, x, y, z)];
+}
+
+__global__ void __launch_bounds__(256) k_heat(HeatArgs h) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int z = blockIdx.z;
+    if (x >= (int)h.nx || y >= (int)h.ny) return;
+    const uint64_t idx = (uint64_t)x + (uint64_t)h.nx * ((uint64_t)y + (uint64_t)h.ny * (uint64_t)z);
+    const float4 fv = h.fields[idx];
+    if (fv.w <= 0.0f) { h.tout[idx] = h.emit[idx]; return; }
+    const float gx = (float)x - fv.x * h.dt, gy = (float)y - fv.y * h.dt, gz = (float)z - fv.z * h.dt;
+    const float flx = __builtin_floorf(gx), fly = __builtin_floorf(gy), flz = __builtin_floorf(gz);
+    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+    const float fx = gx - (float)ix, fy = gy - (float)iy, fz = gz - (float)iz;
+    float t = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+        const float w = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy) * (oz ? fz : 1.0f - fz);
+        t = t + heat_at(h, ix + ox, iy + oy, iz + oz) * w;
+    }
+    const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    float nb = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; k++) nb = nb + heat_at(h, x + off[k][0], y + off[k][1], z + off[k][2]);
+    float src = 0.0f;
+    if (h.flags[idx] & FL_HOTNB) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            const int cx = x + off[k][0], cy = y + off[k][1], cz = z + off[k][2];
+            if (cx >= 0 && cy >= 0 && cx < (int)h.nx && cy < (int)h.ny && cz < (int)h.nz) {
+                const float e = h.emit[heat_idx(h, cx, cy, cz)];
+                src = src > e ? src : e;
+            }
+        }
+    }
+    t = t * (1.0f - h.diffuse) + (nb / 6.0f) * h.diffuse;
+    if (src > 0.0f) t = t * (1.0f - 0.35f) + src * 0.35f;
+    const float o = t * (1.0f - h.cool);
+    h.tout[idx] = o > 0.0f ? o : 0.0f;
+}
+
+hipError_t launch_heat(aero_ctx* c, const aero_heat* hp) {
+    HeatArgs h{};
+    h.fields = c->fields;
+    h.emit = c->emit;
+    h.flags = c->flags;
+    h.tin = c->temps[c->tcur];
+    h.tout = c->temps[c->tcur ^ 1];
+    h.nx = c->grid.nx; h.ny = c->grid.ny; h.nz = c->grid.nz;
+    h.dt = hp->dt; h.cool = hp->cool; h.diffuse = hp->diffuse;
+    const dim3 grid((c->grid.nx + 63) / 64, (c->grid.ny + 3) / 4, c->grid.nz);
+    hipLaunchKernelGGL(k_heat, grid, dim3(256), 0, c->stream, h);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) c->tcur ^= 1;
+    return e;
+}
+
+}  // namespace aero

@@ -1,0 +1,545 @@
+// D3Q19 stream-collide, initialisation and force-reduction kernels for gfx950.
+//
+// Reference behaviour: src/engine/wgsl.ts:141-292 (LBM_INIT, LBM_STEP).  The kernel
+// below is this library's own design for CDNA4:
+//   * SoA populations pulled from x - c_i; each population value of the previous
+//     step is consumed by exactly one cell, so nothing is staged or re-read: a
+//     wavefront covers 64 consecutive x cells of one row, which makes every y/z
+//     boundary branch wave-uniform and every load one contiguous 256-byte run.
+//   * Boundary classification is precomputed into one flag byte per cell; the 18
+//     neighbour groups are read only for cells flagged as touching a solid.
+//   * Tiles are remapped so that each XCD (and its private L2) owns a contiguous
+//     slab of the grid: x-adjacent tiles share the cache lines their shifted
+//     loads straddle.
+//   * Populations are stored as f32, or as f16/bf16 of (f_i - w_i); collision is
+//     f32 in every case, evaluated in the reference's operation order (no FMA
+//     contraction, -ffp-contract=off), so the f32 mode reproduces the reference
+//     arithmetic bit for bit.
+//   * Forces: each cell's link sum is reduced across the wavefront with
+//     cross-lane shuffles in FP64 (and as the reference's 1e4 fixed point), then
+//     added by one thread into the tile's own slot: deterministic, no atomics.
+#include <utility>
+
+#include "aero_internal.h"
+
+namespace aero {
+
+// ---- population storage formats ----
+template <int S>
+struct Store;
+template <>
+struct Store<AERO_STORE_F32> {
+    using T = float;
+    static __device__ __forceinline__ float ld(const T* p, float) { return *p; }
+    static __device__ __forceinline__ void st(T* p, float v, float) { __builtin_nontemporal_store(v, p); }
+    static __host__ __device__ __forceinline__ T enc(float v, float) { return v; }
+};
+template <>
+struct Store<AERO_STORE_F16S> {
+    using T = _Float16;
+    static __device__ __forceinline__ float ld(const T* p, float w) { return (float)(*p) + w; }
+    static __device__ __forceinline__ void st(T* p, float v, float w) {
+        __builtin_nontemporal_store((_Float16)(v - w), p);
+    }
+    static __host__ __device__ __forceinline__ T enc(float v, float w) { return (_Float16)(v - w); }
+};
+__host__ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
+    uint32_t u = __builtin_bit_cast(uint32_t, x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+template <>
+struct Store<AERO_STORE_BF16S> {
+    using T = uint16_t;
+    static __device__ __forceinline__ float ld(const T* p, float w) {
+        return __builtin_bit_cast(float, (uint32_t)(*p) << 16) + w;
+    }
+    static __device__ __forceinline__ void st(T* p, float v, float w) {
+        __builtin_nontemporal_store(f32_to_bf16_rne(v - w), p);
+    }
+    static __host__ __device__ __forceinline__ T enc(float v, float w) { return f32_to_bf16_rne(v - w); }
+};
+
+// Workgroups are dealt round-robin to the 8 XCDs; map them so that XCD k receives
+// the contiguous logical tile range [s_k, s_k + c_k).
+__device__ __forceinline__ uint32_t xcd_swizzle(uint32_t b, uint32_t G) {
+    const uint32_t q = G >> 3, r = G & 7u, k = b & 7u, j = b >> 3;
+    return k * q + (k < r ? k : r) + j;
+}
+
+// c·u summed left to right over the non-zero components of c_I (wgsl.ts:136):
+// the zero terms of the reference's sum are exact no-ops for finite u.
+template <int I>
+__device__ __forceinline__ float cdotu(float ux, float uy, float uz) {
+    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+    float s = 0.0f;
+    bool first = true;
+    if constexpr (cx != 0) { s = cx > 0 ? ux : -ux; first = false; }
+    if constexpr (cy != 0) { float t = cy > 0 ? uy : -uy; s = first ? t : s + t; first = false; }
+    if constexpr (cz != 0) { float t = cz > 0 ? uz : -uz; s = first ? t : s + t; first = false; }
+    return s;
+}
+// feq(I, rho, u) with wr = w_I*rho and uu15 = 1.5*dot(u,u) (wgsl.ts:135-138).
+template <int I>
+__device__ __forceinline__ float feq_c(float wr, float ux, float uy, float uz, float uu15) {
+    if constexpr (I == 0) {
+        return wr * (1.0f - uu15);
+    } else {
+        const float cu = cdotu<I>(ux, uy, uz);
+        return wr * (((1.0f + 3.0f * cu) + (4.5f * cu) * cu) - uu15);
+    }
+}
+__device__ __forceinline__ float dot3(float x, float y, float z) { return (x * x + y * y) + z * z; }
+
+template <typename F, int... Is>
+__device__ __forceinline__ void unroll_seq(F&& f, std::integer_sequence<int, Is...>) {
+    (f(std::integral_constant<int, Is>{}), ...);
+}
+template <typename F>
+__device__ __forceinline__ void for19(F&& f) {
+    unroll_seq(f, std::make_integer_sequence<int, 19>{});
+}
+
+template <int S>
+__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+    using ST = Store<S>;
+    using T = typename ST::T;
+    const T* __restrict__ fin = (const T*)a.fin;
+    T* __restrict__ fout = (T*)a.fout;
+
+    const uint32_t l = xcd_swizzle(blockIdx.x, gridDim.x);
+    const uint32_t txy = a.tiles_x * a.tiles_y;
+    const uint32_t tz = l / txy;
+    const uint32_t rem = l - tz * txy;
+    const uint32_t ty = rem / a.tiles_x;
+    const uint32_t tx = rem - ty * a.tiles_x;
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int x = (int)tx * kTileX + lane;
+    const int y = (int)ty * kTileY + wv;  // wave-uniform
+    const int z = (int)tz;                // block-uniform
+    const int NX = (int)a.nx, NY = (int)a.ny, NZ = (int)a.nz;
+    const uint64_t ns = a.ns;
+    const bool active = (x < NX) && (y < NY);
+
+    // Per-cell momentum-exchange force, groups rest/front/rear x (x,y,z).
+    float g0x = 0.f, g0y = 0.f, g0z = 0.f, g1x = 0.f, g1y = 0.f, g1z = 0.f, g2x = 0.f, g2y = 0.f, g2z = 0.f;
+    bool guard = false;
+
+    if (active) {
+        const uint64_t idx = (uint64_t)x + (uint64_t)NX * ((uint64_t)y + (uint64_t)NY * (uint64_t)z);
+        const uint8_t fl = a.flags[idx];
+        const bool outlet = (x == NX - 1);
+        const uint64_t pidx = ((uint64_t)y + (uint64_t)NY * (uint64_t)z) * 3u;
+        if (fl & FL_GROUP) {
+            // Solid cell: fout_i = w_i, fields = 0 (wgsl.ts:182-186).
+            for19([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                ST::st(fout + (uint64_t)I * ns + idx, wt(I), wt(I));
+            });
+            if (a.write_fields) a.fields[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (outlet) { a.prev_u[pidx] = 0.f; a.prev_u[pidx + 1] = 0.f; a.prev_u[pidx + 2] = 0.f; }
+        } else {
+            const bool near = (fl & FL_NEAR) != 0;
+            const bool duct = (fl & FL_DUCT) != 0;
+            const bool floor0 = (fl & FL_FLOOR0) != 0;
+            float pux = 0.f, puy = 0.f, puz = 0.f, puu15 = 0.f;
+            if (outlet) {
+                pux = a.prev_u[pidx]; puy = a.prev_u[pidx + 1]; puz = a.prev_u[pidx + 2];
+                puu15 = 1.5f * dot3(pux, puy, puz);
+            }
+            const int64_t sxy = (int64_t)NX * (int64_t)NY;
+
+            float f[19];
+            // Pull streaming with the boundary rules of wgsl.ts:203-229, tested in the
+            // reference's order: inlet, outlet, moving floor, far field, symmetry
+            // mirror, solid bounce-back, interior.
+            for19([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                constexpr float w = wt(I);
+                float v = 0.0f;
+                if constexpr (I == 0) {
+                    v = ST::ld(fin + idx, w);
+                } else {
+                    bool done = false;
+                    if constexpr (cx == 1) { if (x == 0) { v = a.feq_in[I]; done = true; } }
+                    if constexpr (cx == -1) {
+                        if (!done && outlet) { v = feq_c<I>(w, pux, puy, puz, puu15); done = true; }
+                    }
+                    if constexpr (cy == 1) {
+                        if (!done && y == 0) {
+                            v = ST::ld(fin + (uint64_t)kOPP[I] * ns + idx, wt(kOPP[I]));
+                            if constexpr (cx != 0) { if (!floor0) v = v + a.floor_term[I]; }
+                            done = true;
+                        }
+                    }
+                    if constexpr (cy == -1) { if (!done && y == NY - 1) { v = a.feq_in[I]; done = true; } }
+                    if constexpr (cz == -1) { if (!done && z == NZ - 1) { v = a.feq_in[I]; done = true; } }
+                    if (!done) {
+                        bool mir = false;
+                        if constexpr (cz == 1) { mir = (z == 0); }
+                        const int64_t off = (int64_t)cx + (int64_t)cy * NX + (mir ? 0 : (int64_t)cz * sxy);
+                        const uint64_t sidx = (uint64_t)((int64_t)idx - off);
+                        uint32_t sg = 0;
+                        if (near) sg = a.flags[sidx] & FL_GROUP;
+                        if (sg != 0) {
+                            v = ST::ld(fin + (uint64_t)kOPP[I] * ns + idx, wt(kOPP[I]));
+                            if (!mir && !duct) {
+                                const float m = 2.0f * (v - w);
+                                // gforce[sg-1] -= m * c  (wgsl.ts:223)
+#define AERO_ACC(G, gx, gy, gz)                                   \
+    if (sg == G) {                                                \
+        if constexpr (cx != 0) gx = (cx > 0) ? gx - m : gx + m;   \
+        if constexpr (cy != 0) gy = (cy > 0) ? gy - m : gy + m;   \
+        if constexpr (cz != 0) gz = (cz > 0) ? gz - m : gz + m;   \
+    }
+                                AERO_ACC(1u, g0x, g0y, g0z)
+                                AERO_ACC(2u, g1x, g1y, g1z)
+                                AERO_ACC(3u, g2x, g2y, g2z)
+#undef AERO_ACC
+                            }
+                        } else {
+                            if constexpr (cz == 1) {
+                                if (mir) v = ST::ld(fin + (uint64_t)kZMIR[I] * ns + sidx, wt(kZMIR[I]));
+                                else v = ST::ld(fin + (uint64_t)I * ns + sidx, w);
+                            } else {
+                                v = ST::ld(fin + (uint64_t)I * ns + sidx, w);
+                            }
+                        }
+                    }
+                }
+                f[I] = v;
+            });
+
+            // Moments (wgsl.ts:231-236).
+            float rho = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
+            bool fx = true, fy = true, fz = true;
+            for19([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                rho = (I == 0) ? f[0] : rho + f[I];
+                if constexpr (cx != 0) { float t = cx > 0 ? f[I] : -f[I]; mx = fx ? t : mx + t; fx = false; }
+                if constexpr (cy != 0) { float t = cy > 0 ? f[I] : -f[I]; my = fy ? t : my + t; fy = false; }
+                if constexpr (cz != 0) { float t = cz > 0 ? f[I] : -f[I]; mz = fz ? t : mz + t; fz = false; }
+            });
+            const float den = rho > 1e-4f ? rho : 1e-4f;
+            float ux = mx / den, uy = my / den, uz = mz / den;
+            // Stability guard (wgsl.ts:238-243).
+            const float ulen = __builtin_sqrtf(dot3(ux, uy, uz));
+            if (!__builtin_isfinite(rho) || !__builtin_isfinite(ux) || !__builtin_isfinite(uy) ||
+                !__builtin_isfinite(uz) || rho < 0.3f || rho > 3.0f || ulen > 0.35f) {
+                rho = 1.0f;
+                ux = a.u_in * 0.5f; uy = 0.0f; uz = 0.0f;
+                const float gu15 = 1.5f * dot3(ux, uy, uz);
+                for19([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    f[I] = feq_c<I>(wt(I) * rho, ux, uy, uz, gu15);
+                });
+                g0x = g0y = g0z = g1x = g1y = g1z = g2x = g2y = g2z = 0.f;
+                guard = true;
+            }
+            // Duct relaxation (wgsl.ts:248): mix(u, (0.55 uIn, u.y, u.z), 0.04).
+            if (duct) {
+                const float t = 0.04f, omt = 1.0f - t;
+                const float bx = 0.55f * a.u_in;
+                ux = ux * omt + bx * t;
+                uy = uy * omt + uy * t;
+                uz = uz * omt + uz * t;
+            }
+            // Non-equilibrium stress (wgsl.ts:250-257).
+            const float uu15 = 1.5f * dot3(ux, uy, uz);
+            float eq[19];
+            float pxx = 0.f, pyy = 0.f, pzz = 0.f, pxy = 0.f, pxz = 0.f, pyz = 0.f;
+            bool bxx = true, byy = true, bzz = true, bxy = true, bxz = true, byz = true;
+            for19([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                eq[I] = feq_c<I>(wt(I) * rho, ux, uy, uz, uu15);
+                const float ne = f[I] - eq[I];
+                if constexpr (cx != 0) { pxx = bxx ? ne : pxx + ne; bxx = false; }
+                if constexpr (cy != 0) { pyy = byy ? ne : pyy + ne; byy = false; }
+                if constexpr (cz != 0) { pzz = bzz ? ne : pzz + ne; bzz = false; }
+                if constexpr (cx * cy != 0) { float t = cx * cy > 0 ? ne : -ne; pxy = bxy ? t : pxy + t; bxy = false; }
+                if constexpr (cx * cz != 0) { float t = cx * cz > 0 ? ne : -ne; pxz = bxz ? t : pxz + t; bxz = false; }
+                if constexpr (cy * cz != 0) { float t = cy * cz > 0 ? ne : -ne; pyz = byz ? t : pyz + t; byz = false; }
+            });
+            // Smagorinsky relaxation time with the inlet/outlet sponge (wgsl.ts:258-262).
+            const float q = __builtin_sqrtf(((pxx * pxx + pyy * pyy) + pzz * pzz) +
+                                            2.0f * ((pxy * pxy + pxz * pxz) + pyz * pyz));
+            const float tau0 = a.tau_x[x];
+            const float tau = 0.5f * (tau0 + __builtin_sqrtf(tau0 * tau0 + a.smag_c * q / rho));
+            const float omega = 1.0f / tau;
+            const float om45 = (1.0f - omega) * 4.5f;
+            // Regularized collision (wgsl.ts:265-271).
+            const float third = (float)(1.0 / 3.0);
+            const float kA = 1.0f - third, kB = 0.0f - third;
+            for19([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                float qp = ((cx != 0 ? kA : kB) * pxx + (cy != 0 ? kA : kB) * pyy) + (cz != 0 ? kA : kB) * pzz;
+                if constexpr (cx * cy != 0) qp = qp + 2.0f * (cx * cy > 0 ? pxy : -pxy);
+                if constexpr (cx * cz != 0) qp = qp + 2.0f * (cx * cz > 0 ? pxz : -pxz);
+                if constexpr (cy * cz != 0) qp = qp + 2.0f * (cy * cz > 0 ? pyz : -pyz);
+                const float v = eq[I] + (om45 * wt(I)) * qp;
+                ST::st(fout + (uint64_t)I * ns + idx, v, wt(I));
+            });
+            if (a.write_fields) a.fields[idx] = make_float4(ux, uy, uz, rho);
+            if (outlet) { a.prev_u[pidx] = ux; a.prev_u[pidx + 1] = uy; a.prev_u[pidx + 2] = uz; }
+        }
+    }
+
+    // ---- force and guard reduction into the tile's slot ----
+    const bool has = guard || g0x != 0.f || g0y != 0.f || g0z != 0.f || g1x != 0.f || g1y != 0.f ||
+                     g1z != 0.f || g2x != 0.f || g2y != 0.f || g2z != 0.f;
+    if (__syncthreads_or(has)) {
+        __shared__ double sd[4][9];
+        __shared__ long long sq[4][6];
+        __shared__ unsigned int sgd[4];
+        double d[9] = {g0x, g0y, g0z, g1x, g1y, g1z, g2x, g2y, g2z};
+        // wgsl.ts:280-282: i32(round(f * 1e4)) when non-zero; WGSL round is half-to-even.
+        const float gq[6] = {g0x, g0y, g1x, g1y, g2x, g2y};
+        long long qv[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) qv[k] = gq[k] != 0.f ? (long long)(int)__builtin_rintf(gq[k] * 10000.0f) : 0;
+        unsigned int gc = guard ? 1u : 0u;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) d[k] += __shfl_xor(d[k], o, 64);
+#pragma unroll
+            for (int k = 0; k < 6; k++) qv[k] += __shfl_xor(qv[k], o, 64);
+            gc += __shfl_xor(gc, o, 64);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 9; k++) sd[wv][k] = d[k];
+#pragma unroll
+            for (int k = 0; k < 6; k++) sq[wv][k] = qv[k];
+            sgd[wv] = gc;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            TileSlot* s = a.slots + l;
+#pragma unroll
+            for (int k = 0; k < 9; k++) s->d[k] += ((sd[0][k] + sd[1][k]) + sd[2][k]) + sd[3][k];
+            const int qmap[6] = {0, 1, 3, 4, 6, 7};
+#pragma unroll
+            for (int k = 0; k < 6; k++) s->q[qmap[k]] += ((sq[0][k] + sq[1][k]) + sq[2][k]) + sq[3][k];
+            s->guard += (unsigned long long)sgd[0] + sgd[1] + sgd[2] + sgd[3];
+        }
+    }
+}
+
+// LBM_INIT (wgsl.ts:147-154) on both buffers: feq(i, 1, (uIn,0,0)) everywhere,
+// fields = (uIn,0,0,1) and the outlet velocity plane = (uIn,0,0).
+template <int S>
+__global__ void __launch_bounds__(256) k_init(StepArgs a, void* fa, void* fb, uint64_t n) {
+    using ST = Store<S>;
+    using T = typename ST::T;
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    T* A = (T*)fa;
+    T* B = (T*)fb;
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        const T v = ST::enc(a.feq_in[i], wt(i));
+        A[(uint64_t)i * a.ns + idx] = v;
+        B[(uint64_t)i * a.ns + idx] = v;
+    }
+    a.fields[idx] = make_float4(a.u_in, 0.f, 0.f, 1.f);
+    const uint32_t x = (uint32_t)(idx % a.nx);
+    if (x == a.nx - 1) {
+        const uint64_t p = (idx / a.nx) * 3u;
+        a.prev_u[p] = a.u_in; a.prev_u[p + 1] = 0.f; a.prev_u[p + 2] = 0.f;
+    }
+}
+
+// Flag byte per cell from solid/emit (see FL_* in aero_internal.h).  The near-solid
+// and floor-contact rules follow the neighbour reads of wgsl.ts:191-198,212-219; the
+// hot-neighbour rule follows the source test of wgsl.ts:500-505.
+__global__ void __launch_bounds__(256) k_flags(const uint32_t* __restrict__ solid, const float* __restrict__ emit,
+                                              uint8_t* __restrict__ flags, uint32_t nx, uint32_t ny, uint32_t nz) {
+    const uint64_t n = (uint64_t)nx * ny * nz;
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const int NX = nx, NY = ny, NZ = nz;
+    const int x = (int)(idx % nx), y = (int)((idx / nx) % ny), z = (int)(idx / ((uint64_t)nx * ny));
+    const uint32_t sg = solid[idx];
+    uint8_t fl = (uint8_t)(sg & 3u);
+    if (sg == 0u) {
+        bool near = false;
+#pragma unroll
+        for (int i = 1; i < 19; i++) {
+            const int sx = x - kCX[i], sy = y - kCY[i];
+            int sz = z - kCZ[i];
+            if (sx < 0 || sx >= NX || sy < 0 || sy >= NY || sz >= NZ) continue;
+            if (sz < 0) sz = 0;
+            if (solid[(uint64_t)sx + (uint64_t)NX * ((uint64_t)sy + (uint64_t)NY * sz)] != 0u) near = true;
+        }
+        if (near) fl |= FL_NEAR;
+        if (emit[idx] < 0.0f) fl |= FL_DUCT;
+        if (y == 0) {
+            const uint64_t row = (uint64_t)z * NX * NY;
+            const int xl = x - 2 > 0 ? x - 2 : 0, xr = x + 2 < NX - 1 ? x + 2 : NX - 1;
+            const int zl = z - 1 > 0 ? z - 1 : 0, zr = z + 1 < NZ - 1 ? z + 1 : NZ - 1;
+            if (solid[row + xl] != 0u || solid[row + xr] != 0u || solid[idx + NX] != 0u ||
+                solid[(uint64_t)zl * NX * NY + x] != 0u || solid[(uint64_t)zr * NX * NY + x] != 0u)
+                fl |= FL_FLOOR0;
+        }
+    }
+    // Heat: max emit over the in-range 6-neighbours (z < 0 mirrors to z = 0).
+    const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    bool hot = false;
+    for (int k = 0; k < 6; k++) {
+        const int cx = x + off[k][0], cy = y + off[k][1];
+        int cz = z + off[k][2];
+        if (cx >= 0 && cy >= 0 && cx < NX && cy < NY && cz < NZ) {
+            if (cz < 0) cz = -cz - 1;
+            if (emit[(uint64_t)cx + (uint64_t)NX * ((uint64_t)cy + (uint64_t)NY * cz)] > 0.0f) hot = true;
+        }
+    }
+    if (hot) fl |= FL_HOTNB;
+    flags[idx] = fl;
+}
+
+// Fixed-order reduction of the tile slots into one TileSlot.
+__global__ void __launch_bounds__(256) k_reduce_slots(const TileSlot* __restrict__ s, uint32_t ntiles, TileSlot* out) {
+    __shared__ double sd[256][9];
+    __shared__ long long sq[256][9];
+    __shared__ unsigned long long sg[256];
+    const int t = threadIdx.x;
+    double d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    long long q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned long long g = 0;
+    for (uint32_t i = t; i < ntiles; i += 256) {
+        for (int k = 0; k < 9; k++) { d[k] += s[i].d[k]; q[k] += s[i].q[k]; }
+        g += s[i].guard;
+    }
+    for (int k = 0; k < 9; k++) { sd[t][k] = d[k]; sq[t][k] = q[k]; }
+    sg[t] = g;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (t < w) {
+            for (int k = 0; k < 9; k++) { sd[t][k] += sd[t + w][k]; sq[t][k] += sq[t + w][k]; }
+            sg[t] += sg[t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        for (int k = 0; k < 9; k++) { out->d[k] = sd[0][k]; out->q[k] = sq[0][k]; }
+        out->guard = sg[0];
+        out->pad = 0;
+    }
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) k_pack(const float* __restrict__ in, void* out, uint64_t n, uint64_t ns) {
+    using ST = Store<S>;
+    using T = typename ST::T;
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    for (int i = 0; i < 19; i++) ((T*)out)[(uint64_t)i * ns + idx] = ST::enc(in[(uint64_t)i * n + idx], wt(i));
+}
+template <int S>
+__global__ void __launch_bounds__(256) k_unpack(const void* in, float* __restrict__ out, uint64_t n, uint64_t ns) {
+    using ST = Store<S>;
+    using T = typename ST::T;
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    for (int i = 0; i < 19; i++) out[(uint64_t)i * n + idx] = ST::ld((const T*)in + (uint64_t)i * ns + idx, wt(i));
+}
+
+// ---- host side ----
+
+// Uniform precomputation, in f32 with the reference's operation order (the host
+// side of this library is also built with -ffp-contract=off).
+static float feq_host(int i, float rho, float ux, float uy, float uz) {
+    const float cu = (float)kCX[i] * ux + (float)kCY[i] * uy + (float)kCZ[i] * uz;
+    return wt(i) * rho * (1.0f + 3.0f * cu + 4.5f * cu * cu - 1.5f * ((ux * ux + uy * uy) + uz * uz));
+}
+
+void fill_step_consts(const aero_ctx* c, StepArgs* a) {
+    a->ns = c->ns;
+    a->nx = c->grid.nx; a->ny = c->grid.ny; a->nz = c->grid.nz;
+    a->tiles_x = c->tiles_x; a->tiles_y = c->tiles_y; a->ntiles = c->ntiles;
+    a->u_in = c->sim.u_in;
+    a->smag_c = (float)(18.0 * 1.41421356) * c->sim.smag * c->sim.smag;
+    for (int i = 0; i < 19; i++) {
+        a->feq_in[i] = feq_host(i, 1.0f, c->sim.u_in, 0.0f, 0.0f);
+        a->floor_term[i] = 6.0f * wt(i) * (float)kCX[i] * c->sim.u_in;
+    }
+    a->flags = c->flags;
+    a->fields = c->fields;
+    a->prev_u = c->prev_u;
+    a->slots = c->slots;
+    a->tau_x = c->tau_x;
+}
+
+hipError_t launch_init(aero_ctx* c) {
+    StepArgs a{};
+    fill_step_consts(c, &a);
+    const uint64_t n = c->grid.n;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    switch (c->storage) {
+    case AERO_STORE_F32: hipLaunchKernelGGL(k_init<AERO_STORE_F32>, grid, dim3(256), 0, c->stream, a, c->f[0], c->f[1], n); break;
+    case AERO_STORE_F16S: hipLaunchKernelGGL(k_init<AERO_STORE_F16S>, grid, dim3(256), 0, c->stream, a, c->f[0], c->f[1], n); break;
+    default: hipLaunchKernelGGL(k_init<AERO_STORE_BF16S>, grid, dim3(256), 0, c->stream, a, c->f[0], c->f[1], n); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_flags(aero_ctx* c) {
+    const uint64_t n = c->grid.n;
+    hipLaunchKernelGGL(k_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->solid, c->emit,
+                       c->flags, c->grid.nx, c->grid.ny, c->grid.nz);
+    return hipGetLastError();
+}
+
+hipError_t launch_step(aero_ctx* c, int write_fields) {
+    StepArgs a{};
+    fill_step_consts(c, &a);
+    a.fin = c->f[c->cur];
+    a.fout = c->f[c->cur ^ 1];
+    a.write_fields = write_fields;
+    const dim3 grid(c->ntiles), block(kBlock);
+    switch (c->storage) {
+    case AERO_STORE_F32: hipLaunchKernelGGL(k_step<AERO_STORE_F32>, grid, block, 0, c->stream, a); break;
+    case AERO_STORE_F16S: hipLaunchKernelGGL(k_step<AERO_STORE_F16S>, grid, block, 0, c->stream, a); break;
+    default: hipLaunchKernelGGL(k_step<AERO_STORE_BF16S>, grid, block, 0, c->stream, a); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) c->cur ^= 1;
+    return e;
+}
+
+hipError_t launch_reduce_slots(aero_ctx* c) {
+    hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->slots, c->ntiles, (TileSlot*)c->red_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_pops(aero_ctx* c, const float* d_in) {
+    const uint64_t n = c->grid.n;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    void* out = c->f[c->cur];
+    switch (c->storage) {
+    case AERO_STORE_F32: hipLaunchKernelGGL(k_pack<AERO_STORE_F32>, grid, dim3(256), 0, c->stream, d_in, out, n, c->ns); break;
+    case AERO_STORE_F16S: hipLaunchKernelGGL(k_pack<AERO_STORE_F16S>, grid, dim3(256), 0, c->stream, d_in, out, n, c->ns); break;
+    default: hipLaunchKernelGGL(k_pack<AERO_STORE_BF16S>, grid, dim3(256), 0, c->stream, d_in, out, n, c->ns); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_pops(aero_ctx* c, float* d_out) {
+    const uint64_t n = c->grid.n;
+    const dim3 grid((unsigned)((n + 255) / 256));
+    const void* in = c->f[c->cur];
+    switch (c->storage) {
+    case AERO_STORE_F32: hipLaunchKernelGGL(k_unpack<AERO_STORE_F32>, grid, dim3(256), 0, c->stream, in, d_out, n, c->ns); break;
+    case AERO_STORE_F16S: hipLaunchKernelGGL(k_unpack<AERO_STORE_F16S>, grid, dim3(256), 0, c->stream, in, d_out, n, c->ns); break;
+    default: hipLaunchKernelGGL(k_unpack<AERO_STORE_BF16S>, grid, dim3(256), 0, c->stream, in, d_out, n, c->ns); break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace aero

@@ -1,0 +1,170 @@
+/*
+ * aero_lbm.h — C ABI of the MI355X (gfx950) wind-tunnel solver library
+ * (libaero_lbm.so).
+ *
+ * It replaces the simulation half of the reference's AeroEngine
+ * (src/engine/engine.ts:159-769), whose WGSL kernels run through vgpu/WebGPU.
+ * Every entry point cites the reference member or kernel it stands in for.
+ *
+ * Conventions
+ *   - Return value: AERO_OK (0) or a negative AERO_E_* code; the message is kept per
+ *     context (aero_last_error) or, for failures before a context exists, globally
+ *     (aero_global_error).
+ *   - The library owns all device memory.  Host pointers are borrowed for the
+ *     duration of the call only.
+ *   - One host thread per context.  Calls are ordered on the context's own HIP
+ *     stream; reads synchronise that stream.  Contexts are independent.
+ *   - Cell index idx = x + nx*(y + ny*z), x fastest (wgsl.ts:187-189).  Population
+ *     arrays crossing the ABI are SoA f32: pop[i*n + idx], i in D3Q19 order
+ *     (wgsl.ts:124-133), de-shifted to plain f32 whatever the storage format.
+ */
+#ifndef AERO_LBM_H
+#define AERO_LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AERO_OK 0
+#define AERO_E_INVALID (-1)  /* bad argument: null pointer, bad dims, bad enum */
+#define AERO_E_NOMEM (-2)    /* device allocation failed */
+#define AERO_E_HIP (-3)      /* HIP runtime error */
+#define AERO_E_STATE (-4)    /* call not valid in the current state */
+
+/* Population storage in HBM.  Collision arithmetic is FP32 in every mode. */
+#define AERO_STORE_F32 0   /* plain f32, 153 B per cell update */
+#define AERO_STORE_F16S 1  /* f16 of (f_i - w_i), 77 B per cell update */
+#define AERO_STORE_BF16S 2 /* bf16 of (f_i - w_i), 77 B per cell update */
+
+/* == WGSL struct Grid (wgsl.ts:6), 32 bytes.  n must equal nx*ny*nz. */
+typedef struct { uint32_t nx, ny, nz, n; float ox, oy, oz, h; } aero_grid;
+/* == WGSL struct Sim (wgsl.ts:142,158), 16 bytes.  smag is the Smagorinsky Cs. */
+typedef struct { float u_in, tau0, smag, pad; } aero_sim;
+/* == WGSL struct Heat (wgsl.ts:461), 16 bytes. */
+typedef struct { float dt, cool, diffuse, pad; } aero_heat;
+/* == WGSL struct Shape (wgsl.ts:10-17), 96 bytes, packed as engine.ts:429-448 does. */
+typedef struct {
+    uint32_t kind, profile_mode, pw, ph;
+    float front_flap, rear_main, rear_flap, ride_h;
+    float aoa, width, axis_y, p_scale;
+    float p_origin[2], heat_brakes, heat_engine;
+    float heat_rad, front_wing, rear_wing, sidepod;
+    float cover, nose, rear_ride, duct;
+} aero_shape;
+
+/* Shape kinds: 0..6 as ShapeKind in engine.ts:30 (0 F1, 1 sphere, 2 cube, 3 wing,
+ * 4 custom profile, 5 W17, 6 empty).  16 is this library's primitive scene. */
+#define AERO_KIND_F1 0
+#define AERO_KIND_SPHERE 1
+#define AERO_KIND_CUBE 2
+#define AERO_KIND_WING 3
+#define AERO_KIND_CUSTOM 4
+#define AERO_KIND_W17 5
+#define AERO_KIND_EMPTY 6
+#define AERO_KIND_SCENE 16
+
+/* One primitive of a kind-16 scene (64 bytes): a union of the reference's SDF
+ * primitives (wgsl.ts:20-76, f1.ts:60-63) with a material id (7 front wing,
+ * 8 rear wing, anything else counts as "rest"), evaluated at (x, y, |z|) either in
+ * the world frame (frame 0) or in the raked body frame of f1.ts:37-46 (frame 1,
+ * rake from shape.ride_h / shape.rear_ride).
+ *   type 1 round box : a = cx,cy,cz, bx,by,bz, r
+ *   type 2 wheel     : a = cx,cy,cz, r, hw                       (sdWheel)
+ *   type 3 wing      : a = le_x,le_y, chord, ang, thick, z0, z1  (sdWing)
+ *   type 4 sphere    : a = cx,cy,cz, r
+ *   type 5 capsule   : a = ax,ay,az, bx,by,bz, r
+ *   type 6 z-cylinder: a = cx,cy,cz, r, hz                       (sdCylZ)
+ *   type 7 emitter   : a = cx,cy,cz, radius, value  (heat source, no geometry) */
+typedef struct { uint32_t type, material, frame, pad; float a[12]; } aero_prim;
+
+/* W17 baked car (kind 5): the shipped f16 grids and the rig uniform.  Layouts are
+ * those of src/engine/car-model.ts:15-23,60-69,151-190. */
+typedef struct {
+    float front[71][4];
+    float hinges[4], angles[4], extra[4], fly[4];
+    float grids[12][4];
+} aero_rig;
+
+typedef struct aero_ctx aero_ctx;
+
+/* Library/ABI version, for loaders. */
+uint32_t aero_version(void);
+/* Message of the last failure that had no context (aero_create). */
+const char* aero_global_error(void);
+/* Message of the last failure on ctx ("" if none). */
+const char* aero_last_error(const aero_ctx* ctx);
+
+/* Allocates the tunnel on the current HIP device (engine.ts:226-246).  Populations
+ * are left uninitialised: call aero_init.  storage is AERO_STORE_*. */
+int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ctx** out);
+int aero_destroy(aero_ctx* ctx);
+
+/* Replaces the Sim uniform (engine.ts:497-501 syncSim). */
+int aero_set_sim(aero_ctx* ctx, const aero_sim* sim);
+
+/* reset(): LBM_INIT on both population buffers (wgsl.ts:141-155, engine.ts:371-381),
+ * temperatures zeroed, force accumulators and the step counter cleared. */
+int aero_init(aero_ctx* ctx);
+
+/* setShape(): VOXELIZE (wgsl.ts:294-322, engine.ts:426-470).  profile is pw*ph f32
+ * (kind 4, nullable otherwise); prims/nprims describe a kind-16 scene (nullable
+ * otherwise).  Populations are not reset, as in the reference. */
+int aero_voxelize(aero_ctx* ctx, const aero_shape* shape, const float* profile,
+                  const aero_prim* prims, uint32_t nprims);
+
+/* setCarModel() (engine.ts:410-422): uploads the W17 body grid (368x84x67 f16,
+ * public/models/w17.sdf.bin) and the parts data (public/models/w17.parts.bin) with
+ * the rig uniform (car-model.ts:151-171).  parts_atlas is the partsAtlas() packing
+ * (car-model.ts:174-190) with dims atlas_dims. */
+int aero_set_car_model(aero_ctx* ctx, const uint16_t* body_sdf, const uint16_t* parts_atlas,
+                       const uint32_t atlas_dims[3], const aero_rig* rig);
+
+/* Test/driver hook: replaces solid (u32 groups 0..3) and emit (f32) directly. */
+int aero_set_flags(aero_ctx* ctx, const uint32_t* solid, const float* emit);
+
+/* nsteps fused stream-collide steps (LBM_STEP, wgsl.ts:157-292; the frame loop of
+ * engine.ts:708-710).  Any count is allowed; the library tracks the ping-pong. */
+int aero_step(aero_ctx* ctx, uint32_t nsteps);
+
+/* As aero_step, and also returns the GPU time of those steps measured with HIP
+ * events on the context's stream. */
+int aero_step_timed(aero_ctx* ctx, uint32_t nsteps, float* ms);
+
+/* Live timing of the stream-collide kernel: between begin and end, each aero_step
+ * call (up to max_calls of them) is bracketed by a HIP event pair on the context
+ * stream; end synchronises and returns the summed GPU time and the step count.
+ * No reference counterpart (the reference prints wall-clock ms per frame only,
+ * scripts/headless-check.ts:45-50). */
+int aero_profile_begin(aero_ctx* ctx, uint32_t max_calls);
+int aero_profile_end(aero_ctx* ctx, float* ms, uint64_t* steps);
+
+/* nsteps semi-Lagrangian heat steps (HEAT, wgsl.ts:460-511; engine.ts:659-660,711-712). */
+int aero_heat_step(aero_ctx* ctx, const aero_heat* heat, uint32_t nsteps);
+
+/* Force readback (engine.ts:739-761 maybeReadForces).  Slot g*3+k, g = 0 rest,
+ * 1 front wing, 2 rear wing; k = 0 x, 1 y, 2 z.
+ *   sum_fp64: FP64 sums of the per-cell lattice force since the last reset (nullable)
+ *   sum_q:    the reference's fixed-point sums, round(f*1e4) per cell and step,
+ *             x and y only (wgsl.ts:277-290) (nullable)
+ *   steps:    steps accumulated since the last reset (nullable)
+ *   reset:    nonzero zeroes the accumulators afterwards (engine.ts:747). */
+int aero_read_forces(aero_ctx* ctx, double sum_fp64[9], int64_t sum_q[9], uint64_t* steps, int reset);
+
+/* debugRead() (engine.ts:763-768) and state access. */
+int aero_read_fields(aero_ctx* ctx, float* out_n4);     /* (ux,uy,uz,rho) per cell, 0 in solids */
+int aero_read_flags(aero_ctx* ctx, uint32_t* solid, float* emit);
+int aero_read_temps(aero_ctx* ctx, float* out_n);
+int aero_write_temps(aero_ctx* ctx, const float* in_n);
+int aero_read_populations(aero_ctx* ctx, float* out_19n);       /* newest buffer */
+int aero_write_populations(aero_ctx* ctx, const float* in_19n); /* newest buffer */
+/* Number of cells where the stability guard (wgsl.ts:238-243) fired since reset. */
+int aero_guard_count(aero_ctx* ctx, uint64_t* count);
+int aero_sync(aero_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AERO_LBM_H */

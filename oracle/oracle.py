@@ -1,0 +1,175 @@
+"""CPU ORACLE wrapper — test infrastructure only.
+
+Loads the C restatement of the reference kernels (lbm_oracle.c, sdf_oracle.c,
+built by oracle/Makefile into oracle/lib/) and exposes numpy-level helpers.
+Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg import this.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIBDIR = HERE / "lib"
+
+
+def build() -> None:
+    r = subprocess.run(["make", "-C", str(HERE), "-B", "all"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("oracle build failed:\n" + r.stdout + r.stderr)
+
+
+class OGrid(C.Structure):
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("n", C.c_uint32),
+                ("ox", C.c_float), ("oy", C.c_float), ("oz", C.c_float), ("h", C.c_float)]
+
+
+class OSim(C.Structure):
+    _fields_ = [("uIn", C.c_float), ("tau0", C.c_float), ("smag", C.c_float), ("pad", C.c_float)]
+
+
+class OHeat(C.Structure):
+    _fields_ = [("dt", C.c_float), ("cool", C.c_float), ("diffuse", C.c_float), ("pad", C.c_float)]
+
+
+class OStats(C.Structure):
+    _fields_ = [("forces_q", C.c_int64 * 12), ("forces_d", C.c_double * 9), ("guard_hits", C.c_uint64)]
+
+
+class OShape(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("profileMode", C.c_uint32), ("pw", C.c_uint32), ("ph", C.c_uint32),
+                ("frontFlap", C.c_float), ("rearMain", C.c_float), ("rearFlap", C.c_float), ("rideH", C.c_float),
+                ("aoa", C.c_float), ("width", C.c_float), ("axisY", C.c_float), ("pScale", C.c_float),
+                ("pOrigin", C.c_float * 2), ("heatBrakes", C.c_float), ("heatEngine", C.c_float),
+                ("heatRad", C.c_float), ("frontWing", C.c_float), ("rearWing", C.c_float), ("sidepod", C.c_float),
+                ("cover", C.c_float), ("nose", C.c_float), ("rearRide", C.c_float), ("duct", C.c_float)]
+
+
+class OPrim(C.Structure):
+    _fields_ = [("type", C.c_uint32), ("material", C.c_uint32), ("frame", C.c_uint32), ("pad", C.c_uint32),
+                ("a", C.c_float * 12)]
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def grid(dims, h=1.0, origin=(0.0, 0.0, 0.0)) -> OGrid:
+    nx, ny, nz = (int(v) for v in dims)
+    return OGrid(nx, ny, nz, nx * ny * nz, origin[0], origin[1], origin[2], h)
+
+
+def as_shape(s) -> OShape:
+    """Copies any ctypes structure with the Shape layout into an OShape."""
+    o = OShape()
+    C.memmove(C.byref(o), C.byref(s), C.sizeof(OShape))
+    return o
+
+
+def as_prims(prims):
+    if not prims:
+        return None, 0
+    arr = (OPrim * len(prims))()
+    for i, p in enumerate(prims):
+        C.memmove(C.byref(arr[i]), C.byref(p), C.sizeof(OPrim))
+    return arr, len(prims)
+
+
+class LBMOracle:
+    """f32 ('f32'), f64 ('f64') or OpenMP f32 ('f32_omp') restatement of LBM_INIT /
+    LBM_STEP / HEAT."""
+
+    def __init__(self, kind: str = "f32"):
+        path = LIBDIR / f"liboracle_{kind}.so"
+        if not path.exists():
+            build()
+        self.lib = C.CDLL(str(path))
+        self.dtype = np.float32 if kind.startswith("f32") else np.float64
+        R = C.c_float if self.dtype == np.float32 else C.c_double
+        self._R = R
+        L = self.lib
+        L.orc_feq.restype = R
+        L.orc_feq.argtypes = [C.c_uint32, R, R, R, R]
+        L.orc_init.argtypes = [C.POINTER(OGrid), C.POINTER(OSim), C.c_void_p, C.c_void_p]
+        L.orc_run.argtypes = [C.POINTER(OGrid), C.POINTER(OSim), C.c_void_p, C.c_void_p, C.c_void_p,
+                              C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(OStats)]
+        L.orc_heat.argtypes = [C.POINTER(OGrid), C.POINTER(OHeat), C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_tables.argtypes = [C.c_void_p] * 6
+
+    def tables(self):
+        cx = np.zeros(19, np.int32); cy = np.zeros(19, np.int32); cz = np.zeros(19, np.int32)
+        opp = np.zeros(19, np.uint32); zm = np.zeros(19, np.uint32); wt = np.zeros(19, self.dtype)
+        self.lib.orc_tables(_p(cx), _p(cy), _p(cz), _p(opp), _p(zm), _p(wt))
+        return cx, cy, cz, opp, zm, wt
+
+    def feq(self, i, rho, ux, uy, uz):
+        return self.lib.orc_feq(i, rho, ux, uy, uz)
+
+    def init(self, g: OGrid, sim: OSim):
+        f = np.empty((19, g.n), self.dtype)
+        fields = np.empty((g.n, 4), self.dtype)
+        self.lib.orc_init(C.byref(g), C.byref(sim), _p(f), _p(fields))
+        return f, fields
+
+    def run(self, g: OGrid, sim: OSim, f: np.ndarray, solid: np.ndarray, fields: np.ndarray,
+            emit: np.ndarray, nsteps: int):
+        """Advances f (19, n) in place by nsteps; returns (f, fields, stats)."""
+        fa = np.ascontiguousarray(f, dtype=self.dtype).copy()
+        fb = np.empty_like(fa)
+        fl = np.ascontiguousarray(fields, dtype=self.dtype).copy()
+        s = np.ascontiguousarray(solid, dtype=np.uint32)
+        e = np.ascontiguousarray(emit, dtype=np.float32)
+        st = OStats()
+        self.lib.orc_run(C.byref(g), C.byref(sim), _p(fa), _p(fb), _p(s), _p(fl), _p(e), int(nsteps), C.byref(st))
+        out = fb if nsteps % 2 == 1 else fa
+        return out, fl, st
+
+    def heat(self, g: OGrid, hp: OHeat, fields, emit, tin):
+        tout = np.empty(g.n, self.dtype)
+        fl = np.ascontiguousarray(fields, dtype=self.dtype)
+        em = np.ascontiguousarray(emit, dtype=np.float32)
+        ti = np.ascontiguousarray(tin, dtype=self.dtype)
+        self.lib.orc_heat(C.byref(g), C.byref(hp), _p(fl), _p(em), _p(ti), _p(tout))
+        return tout
+
+
+class SDFOracle:
+    """f32 restatement of VOXELIZE and the SDF primitives."""
+
+    def __init__(self):
+        path = LIBDIR / "libsdf_oracle.so"
+        if not path.exists():
+            build()
+        L = self.lib = C.CDLL(str(path))
+        L.orc_voxelize.argtypes = [C.POINTER(OGrid), C.POINTER(OShape), C.c_void_p, C.c_void_p, C.c_uint32,
+                                   C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_sdf_point.restype = C.c_float
+        L.orc_sdf_point.argtypes = [C.POINTER(OShape), C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float,
+                                    C.c_float, C.POINTER(C.c_float)]
+        for name, nargs in (("orc_sd_wing", 10), ("orc_sd_wheel", 8), ("orc_sd_roundbox", 7)):
+            fn = getattr(L, name)
+            fn.restype = C.c_float
+            fn.argtypes = [C.c_float] * nargs
+
+    def voxelize(self, g: OGrid, shape, profile=None, prims=None):
+        sh = as_shape(shape)
+        arr, np_ = as_prims(prims)
+        solid = np.empty(g.n, np.uint32)
+        emit = np.empty(g.n, np.float32)
+        sdf = np.empty(g.n, np.float32)
+        prof = np.ascontiguousarray(profile, dtype=np.float32) if profile is not None else None
+        self.lib.orc_voxelize(C.byref(g), C.byref(sh), _p(prof), C.cast(arr, C.c_void_p) if arr else None, np_,
+                              _p(solid), _p(emit), _p(sdf))
+        return solid, emit, sdf
+
+    def sdf_point(self, shape, x, y, z, profile=None, prims=None):
+        sh = as_shape(shape)
+        arr, np_ = as_prims(prims)
+        m = C.c_float()
+        prof = np.ascontiguousarray(profile, dtype=np.float32) if profile is not None else None
+        d = self.lib.orc_sdf_point(C.byref(sh), _p(prof), C.cast(arr, C.c_void_p) if arr else None, np_,
+                                   x, y, z, C.byref(m))
+        return d, m.value

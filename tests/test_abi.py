@@ -1,0 +1,81 @@
+"""CPU checks of the C ABI: the library loads, exports every function declared in
+include/aero_lbm.h, its structs have the WGSL uniform sizes, and argument errors
+are reported without touching a GPU."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import pytest
+
+import aerolbm
+from aerolbm import _lib
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADERS = sorted((ROOT / "include").glob("*.h"))
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b(aero_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert "aero_step" in names and "aero_voxelize" in names
+    out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True, text=True).stdout
+    exported = {ln.split()[-1] for ln in out.splitlines() if ln.strip()}
+    missing = names - exported
+    assert not missing, missing
+    for n in names:
+        getattr(lib, n)
+    assert set(_lib.exported_symbols()) <= names
+
+
+def test_struct_layouts():
+    assert C.sizeof(_lib.Grid) == 32
+    assert C.sizeof(_lib.Sim) == 16
+    assert C.sizeof(_lib.Heat) == 16
+    assert C.sizeof(_lib.Shape) == 96
+    assert C.sizeof(_lib.Prim) == 64
+    assert C.sizeof(_lib.Rig) == (71 + 4 + 12) * 16
+    assert _lib.Shape.p_origin.offset == 48 and _lib.Shape.duct.offset == 92
+
+
+def test_create_rejects_bad_arguments_before_any_gpu_call():
+    lib = _lib.load()
+    ctx = C.c_void_p()
+    sim = _lib.Sim(0.05, 0.51, 0.1, 0.0)
+    bad_n = _lib.Grid(16, 8, 8, 999, 0, 0, 0, 0.1)
+    assert lib.aero_create(C.byref(bad_n), C.byref(sim), 0, C.byref(ctx)) == -1
+    assert b"grid.n" in lib.aero_global_error()
+    small = _lib.Grid(2, 8, 8, 128, 0, 0, 0, 0.1)
+    assert lib.aero_create(C.byref(small), C.byref(sim), 0, C.byref(ctx)) == -1
+    g = _lib.Grid(16, 8, 8, 1024, 0, 0, 0, 0.1)
+    assert lib.aero_create(C.byref(g), C.byref(sim), 7, C.byref(ctx)) == -1
+    assert b"storage" in lib.aero_global_error()
+    bad_tau = _lib.Sim(0.05, 0.5, 0.1, 0.0)
+    assert lib.aero_create(C.byref(g), C.byref(bad_tau), 0, C.byref(ctx)) == -1
+    assert lib.aero_create(None, C.byref(sim), 0, C.byref(ctx)) == -1
+    assert lib.aero_step(None, 1) == -1
+    assert lib.aero_last_error(None) == b"null context"
+    with pytest.raises(aerolbm.AeroError, match="grid.n"):
+        _lib.check(lib.aero_create(C.byref(bad_n), C.byref(sim), 0, C.byref(ctx)), None, "aero_create")
+
+
+def test_shape_uniform_packing_matches_engine():
+    """engine.ts:429-448 packing of ShapeParams."""
+    p = aerolbm.ShapeParams(kind=0, frontFlapDeg=2, rearFlapDeg=-3, rideH=0.11, rearRideH=0.13, aoaDeg=5,
+                            cooling=2, heatBrakes=True, heatEngine=False, heatRad=True)
+    s = aerolbm.shape_uniform(p)
+    import math
+    assert s.front_flap == pytest.approx((2 + 14) * math.pi / 180, rel=1e-7)
+    assert s.rear_main == pytest.approx(9 * math.pi / 180, rel=1e-7)
+    assert s.rear_flap == pytest.approx((-3 + 32) * math.pi / 180, rel=1e-7)
+    assert s.heat_brakes == 520 and s.heat_engine == 0 and s.heat_rad == pytest.approx(75 * 0.6)
+    assert s.duct == pytest.approx(1.3) and s.pw == 256 and s.ph == 128 and s.p_scale == pytest.approx(0.02)

@@ -1,0 +1,241 @@
+"""GPU parity: libaero_lbm.so (through its C ABI) against the CPU oracle on the same
+seeded inputs.  FP32 storage is expected to reproduce the f32 restatement of the
+reference kernels bit for bit (the tolerance written in each test is the
+north-star bound, 1e-5 relative); FP16/BF16-shifted storage is bounded against the
+FP64 oracle."""
+import math
+
+import numpy as np
+import pytest
+
+import aerolbm
+from aerolbm import workloads as W
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+F32 = orc.LBMOracle("f32")
+F32P = orc.LBMOracle("f32_omp")
+F64 = orc.LBMOracle("f64")
+SDF = orc.SDFOracle()
+REL = 1e-5
+
+
+def engine_for(dims, h=0.1, origin=(0.0, 0.0, 0.0), u=0.05, tau0=0.51, smag=0.1, storage="f32"):
+    return aerolbm.HipAeroEngine(dims=dims, h=h, origin=origin, u_lb=u, tau0=tau0, smag=smag, storage=storage)
+
+
+def ogrid(e):
+    return orc.grid(e.dims, e.h, e.origin)
+
+
+def osim(e):
+    return orc.OSim(e.u_lb, e.tau0, e.smag, 0.0)
+
+
+def rel_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-30)))
+
+
+def mixed_geometry(dims, rng):
+    """Solid boxes of all three groups, a floor-touching block, a body on the symmetry
+    plane and a band of duct cells (emit < 0), plus two heat sources."""
+    nx, ny, nz = dims
+    s = np.zeros((nz, ny, nx), np.uint32)
+    e = np.zeros((nz, ny, nx), np.float32)
+    s[0:4, 3:8, 10:15] = 1
+    s[2:5, 10:12, 20:26] = 2
+    s[5:8, 12:14, 28:31] = 3
+    s[3:6, 0:2, 34:37] = 1          # touches the floor
+    e[8:10, 4:6, 14:22] = -1.0       # duct cells
+    e[1, 5, 12] = 420.0              # heat sources inside solids
+    e[3, 11, 22] = 150.0
+    return s.reshape(-1), e.reshape(-1)
+
+
+def perturbed_init(e, rng, amp=0.02):
+    f0, fields0 = F32.init(ogrid(e), osim(e))
+    f = (f0 * (1 + amp * rng.standard_normal(f0.shape))).astype(np.float32)
+    return f, fields0
+
+
+@pytest.mark.parametrize("steps", [1, 100])
+def test_step_matches_f32_oracle_mixed_geometry(steps):
+    rng = np.random.default_rng(10)
+    dims = (48, 20, 12)
+    e = engine_for(dims, u=0.06)
+    solid, emit = mixed_geometry(dims, rng)
+    e.set_flags(solid, emit)
+    e.reset()
+    f, fields0 = perturbed_init(e, rng)
+    e.write_populations(f)
+    e.step(steps)
+    got = e.populations()
+    want, wfields, st = F32.run(ogrid(e), osim(e), f, solid, fields0, emit, steps)
+    print(f"steps={steps} max rel err {rel_err(got, want):.3e}, exact={np.array_equal(got, want)}")
+    assert rel_err(got, want) <= REL
+    assert np.array_equal(got, want)  # the f32 arithmetic is the reference's, term for term
+    gf = e.fields()
+    assert rel_err(gf[:, 3], wfields[:, 3]) <= REL
+    assert np.abs(gf[:, :3] - wfields[:, :3]).max() <= 1e-7
+    d, q, nsteps = e.read_forces_raw(reset=False)
+    assert nsteps == steps
+    assert np.array_equal(q, np.array(st.forces_q)[:9])
+    od = np.array(st.forces_d)
+    assert np.abs(d - od).max() <= 1e-9 * max(1.0, np.abs(od).max())
+    assert np.abs(od[[0, 3, 6]]).min() > 0  # all three groups carry force
+    assert e.guard_count() == st.guard_hits
+
+
+def test_init_matches_oracle():
+    e = engine_for((40, 12, 9), u=0.0925)
+    e.reset()
+    f0, fields0 = F32.init(ogrid(e), osim(e))
+    assert np.array_equal(e.populations(), f0)
+    assert np.array_equal(e.fields(), fields0)
+    assert not e.temps().any()
+
+
+@pytest.mark.parametrize("steps", [1, 100])
+def test_cube_workload_full_grid(steps):
+    """The 128x64x64 cube workload: voxelize on the GPU, step, compare with the
+    oracle voxelizer + f32 oracle (OpenMP build; per-cell arithmetic is identical)."""
+    wl = W.cube()
+    e = aerolbm.HipAeroEngine(**wl.engine_kwargs())
+    e.set_shape(wl.shape)
+    e.reset()
+    solid, emit = e.flags()
+    osolid, oemit, _ = SDF.voxelize(ogrid(e), wl.shape)
+    assert np.array_equal(solid, osolid) and np.array_equal(emit, oemit)
+    assert int((solid != 0).sum()) == 22 * 22 * 11
+    e.step(steps)
+    f0, fields0 = F32P.init(ogrid(e), osim(e))
+    want, wfields, st = F32P.run(ogrid(e), osim(e), f0, osolid, fields0, oemit, steps)
+    got = e.populations()
+    print(f"cube steps={steps} max rel err {rel_err(got, want):.3e}")
+    assert rel_err(got, want) <= REL
+    d, q, _ = e.read_forces_raw()
+    assert np.array_equal(q, np.array(st.forces_q)[:9])
+    assert abs(d[0] - st.forces_d[0]) <= 1e-9 * abs(st.forces_d[0])
+
+
+@pytest.mark.parametrize("kind", ["cube", "sphere", "wing", "empty", "custom", "scene"])
+def test_voxelize_bit_exact(kind):
+    dims = (120, 48, 40)
+    h = 0.05
+    e = engine_for(dims, h=h, origin=(-0.5, 0.0, 0.0))
+    prof = None
+    prims = None
+    if kind == "cube":
+        s = W.base_shape(aerolbm.ShapeKind.Cube)
+    elif kind == "sphere":
+        s = W.base_shape(aerolbm.ShapeKind.Sphere)
+    elif kind == "wing":
+        s = W.base_shape(aerolbm.ShapeKind.Wing)
+        s.aoa = math.radians(8)
+    elif kind == "empty":
+        s = W.base_shape(aerolbm.ShapeKind.Empty)
+    elif kind == "custom":
+        s = W.base_shape(aerolbm.ShapeKind.Custom)
+        gx = (np.arange(256) + 0.5) * 0.02
+        gy = (np.arange(128) + 0.5) * 0.02
+        X, Y = np.meshgrid(gx, gy)
+        prof = (np.hypot(X - 2.0, Y - 0.8) - 0.35).astype(np.float32).reshape(-1)
+        s.width = 0.3
+        e.set_profile(aerolbm.CustomProfile(prof, 0.0, 0.0, 0.02))
+        s.p_scale = 0.02
+    else:
+        s = W.base_shape(aerolbm.ShapeKind.Scene, ride_h=0.03, rear_ride=0.08)
+        prims = W.generic_f1_prims(0, 0.03)
+    e.set_shape(s, prims=prims)
+    solid, emit = e.flags()
+    osolid, oemit, _ = SDF.voxelize(ogrid(e), s, profile=prof, prims=prims)
+    assert np.array_equal(solid, osolid)
+    assert np.array_equal(emit, oemit)
+    if kind == "scene":
+        assert set(np.unique(solid)) == {0, 1, 2, 3} and (emit > 0).sum() > 0
+    if kind == "empty":
+        assert not solid.any()
+
+
+def test_heat_matches_oracle_after_steps():
+    rng = np.random.default_rng(11)
+    dims = (48, 20, 12)
+    e = engine_for(dims, u=0.08)
+    solid, emit = mixed_geometry(dims, rng)
+    e.set_flags(solid, emit)
+    e.reset()
+    f, fields0 = perturbed_init(e, rng, amp=0.01)
+    e.write_populations(f)
+    t0 = rng.uniform(0, 50, e.n).astype(np.float32)
+    e.write_temps(t0)
+    e.step(6)
+    e.heat_step(2, dt=3.0)
+    _, wfields, _ = F32.run(ogrid(e), osim(e), f, solid, fields0, emit, 6)
+    hp = orc.OHeat(3.0, 0.004, 0.04, 0.0)
+    t1 = F32.heat(ogrid(e), hp, wfields, emit, t0)
+    t2 = F32.heat(ogrid(e), hp, wfields, emit, t1)
+    got = e.temps()
+    print(f"heat max abs err {np.abs(got - t2).max():.3e}")
+    assert np.abs(got - t2).max() <= REL * max(1.0, np.abs(t2).max())
+    assert np.array_equal(got, t2)
+
+
+def test_guard_fires_on_injected_nan():
+    dims = (32, 12, 10)
+    e = engine_for(dims, u=0.06)
+    e.reset()
+    f = e.populations()
+    x, y, z = 16, 6, 5
+    idx = x + dims[0] * (y + dims[1] * z)
+    f[0, idx] = np.nan
+    e.write_populations(f)
+    e.step(1)
+    got = e.populations()
+    want = np.array([F32.feq(i, 1.0, np.float32(0.06) * np.float32(0.5), 0.0, 0.0) for i in range(19)], np.float32)
+    assert np.array_equal(got[:, idx], want)
+    assert e.guard_count() == 1
+    assert np.isfinite(got).all()
+
+
+@pytest.mark.parametrize("storage,bound", [("f16s", 2e-3), ("bf16s", 1.5e-2)])
+def test_shifted_storage_error_vs_fp64(storage, bound):
+    """f16/bf16 of (f - w) with f32 collision, 200 steps of the cube workload at a
+    reduced grid: population and force error against the FP64 oracle."""
+    dims = (64, 32, 24)
+    h = 0.1
+    e = engine_for(dims, h=h, origin=(0.6, 0.0, 0.0), u=0.05, storage=storage)
+    s = W.base_shape(aerolbm.ShapeKind.Cube)
+    e.set_shape(s)
+    e.reset()
+    steps = 200
+    e.step(steps)
+    solid, emit = e.flags()
+    g = ogrid(e)
+    f0, fields0 = F64.init(g, osim(e))
+    want, _, st = F64.run(g, osim(e), f0, solid, fields0, emit, steps)
+    got = e.populations().astype(np.float64)
+    perr = np.abs(got - want).max() / np.abs(want).max()
+    d, _, _ = e.read_forces_raw()
+    ferr = abs(d[0] - st.forces_d[0]) / abs(st.forces_d[0])
+    print(f"{storage}: population err {perr:.2e} (of max), drag-force err {ferr:.2e}")
+    assert perr < bound
+    assert ferr < 0.01
+
+
+def test_forces_reset_and_step_count():
+    e = engine_for((40, 16, 12), u=0.05)
+    s = W.base_shape(aerolbm.ShapeKind.Scene)
+    prims = [W.prim(W.PRIM_SPHERE, 7, 0, 1.5, 0.6, 0.0, 0.3)]
+    e.set_shape(s, prims=prims)
+    e.reset()
+    e.step(5)
+    d1, q1, n1 = e.read_forces_raw(reset=True)
+    assert n1 == 5 and d1[3] != 0 and q1[3] != 0
+    d2, q2, n2 = e.read_forces_raw(reset=False)
+    assert n2 == 0 and not d2.any() and not q2.any()
+    e.step(3)
+    f = e.read_forces(reset=True)
+    assert f is not None and f.frontWing.drag != 0

@@ -1,0 +1,165 @@
+"""Pins the FP32 voxelizer oracle (oracle/sdf_oracle.c) against geometry computed
+independently: closed-form cell sets for the cube and sphere, the convex-hull
+distance of the wing section, clamp-projection distances of the rounded box and
+wheel, and an analytic circle through the custom-profile path."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+SDF = orc.SDFOracle()
+
+
+def shape(kind, **kw):
+    s = orc.OShape()
+    s.kind = kind
+    s.pw, s.ph, s.pScale = 256, 128, 0.02
+    s.rideH, s.rearRide = kw.get("rideH", 0.10), kw.get("rearRide", 0.12)
+    s.aoa = kw.get("aoa", 6 * math.pi / 180)
+    for k, v in kw.items():
+        if hasattr(s, k) and k not in ("rideH", "rearRide", "aoa"):
+            setattr(s, k, v)
+    return s
+
+
+def centres(g):
+    nx, ny, nz = g.nx, g.ny, g.nz
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    f = np.float32
+    px = f(g.ox) + (x.astype(f) + f(0.5)) * f(g.h)
+    py = f(g.oy) + (y.astype(f) + f(0.5)) * f(g.h)
+    pz = f(g.oz) + (z.astype(f) + f(0.5)) * f(g.h)
+    return x.reshape(-1), px.reshape(-1).astype(np.float64), py.reshape(-1).astype(np.float64), pz.reshape(-1).astype(np.float64)
+
+
+def test_cube_workload_is_22_cells_per_side():
+    h = float(np.float32(1 / 22))
+    ox = float(np.float32(2.6 - 42 / 22))
+    g = orc.grid((128, 64, 64), h, (ox, 0.0, 0.0))
+    solid, emit, _ = SDF.voxelize(g, shape(2))
+    s3 = solid.reshape(64, 64, 128)
+    zs, ys, xs = np.nonzero(s3)
+    assert (xs.min(), xs.max(), ys.min(), ys.max(), zs.min(), zs.max()) == (31, 52, 11, 32, 0, 10)
+    assert solid.sum() == 22 * 22 * 11 and set(np.unique(solid)) == {0, 1}
+    # closed form: exactly the cells whose centre lies inside the 1 m box
+    _, px, py, pz = centres(g)
+    inside = (np.abs(px - 2.6) < 0.5) & (np.abs(py - 1.0) < 0.5) & (np.abs(pz) < 0.5)
+    assert np.array_equal(solid != 0, inside)
+    assert not emit.any()
+
+
+def test_sphere_matches_closed_form_outside_band():
+    g = orc.grid((96, 48, 40), 0.05, (0.0, 0.0, 0.0))
+    solid, _, _ = SDF.voxelize(g, shape(1))
+    x, px, py, pz = centres(g)
+    d = np.sqrt((px - 2.6) ** 2 + (py - 1.0) ** 2 + pz ** 2) - 0.6
+    thr = 0.3 * 0.05
+    want = (d < thr) & (x > 1) & (x < 96 - 2)
+    band = np.abs(d - thr) < 1e-5
+    assert np.array_equal((solid != 0)[~band], want[~band])
+    assert band.sum() < 10
+
+
+def hull_dist(p2, c1, c2, r1, r2, nt=20001):
+    """Distance from p2 (N,2) to the convex hull of two discs = min_t |p - c(t)| - r(t)."""
+    t = np.linspace(0, 1, nt)
+    c = c1[None] + (c2 - c1)[None] * t[:, None]
+    r = r1 + (r2 - r1) * t
+    d = np.sqrt(((p2[:, None, :] - c[None]) ** 2).sum(-1)) - r[None]
+    return d.min(1)
+
+
+@pytest.mark.parametrize("ang", [math.radians(4), math.radians(11), math.radians(-6)])
+def test_wing_section_matches_convex_hull_distance(ang):
+    le = np.array([1.6, 1.2])
+    chord, thick, z0, z1 = 0.9, 0.06, -0.5, 0.5
+    r1 = thick * 0.5
+    r2 = max(thick * 0.12, 0.006)
+    hh = max(chord - r1 - r2, 0.01)
+    dirv = np.array([math.cos(ang), math.sin(ang)])
+    c1 = le + r1 * dirv
+    c2 = le + (r1 + hh) * dirv
+    rng = np.random.default_rng(5)
+    pts = np.column_stack([rng.uniform(1.4, 2.7, 400), rng.uniform(0.9, 1.6, 400), rng.uniform(-0.3, 0.3, 400)])
+    d2 = hull_dist(pts[:, :2], c1, c2, r1, r2)
+    dz = np.abs(pts[:, 2]) - 0.5
+    exact = np.minimum(np.maximum(d2, dz), 0) + np.sqrt(np.maximum(d2, 0) ** 2 + np.maximum(dz, 0) ** 2)
+    got = np.array([SDF.lib.orc_sd_wing(*p, le[0], le[1], chord, ang, thick, z0, z1) for p in pts])
+    out = exact > 1e-3
+    assert out.sum() > 100
+    assert np.abs(got[out] - exact[out]).max() < 2e-5
+    # inside points are negative
+    assert np.all(got[exact < -2e-3] < 0)
+    # positive angle raises the trailing edge (downforce sense)
+    te = le + chord * dirv
+    assert (te[1] > le[1]) == (ang > 0)
+
+
+def test_roundbox_and_wheel_match_clamp_projection():
+    rng = np.random.default_rng(6)
+    b = np.array([0.5, 0.3, 0.4])
+    r = 0.05
+    pts = rng.uniform(-1, 1, (500, 3))
+    inner = b - r
+    q = np.clip(pts, -inner, inner)
+    exact = np.sqrt(((pts - q) ** 2).sum(1)) - r
+    got = np.array([SDF.lib.orc_sd_roundbox(*p, *b, r) for p in pts])
+    out = exact > 0
+    assert np.abs(got[out] - exact[out]).max() < 2e-6
+    # wheel: rounded cylinder of radius R, half width hw, edge radius 0.045
+    R, hw, rr = 0.36, 0.15, 0.045
+    c = np.array([1.0, 0.36, 1.12])
+    pts = c + rng.uniform(-0.7, 0.7, (500, 3))
+    rho = np.hypot(pts[:, 0] - c[0], pts[:, 1] - c[1])
+    az = np.abs(pts[:, 2] - c[2])
+    qr = np.clip(rho, 0, R - rr)
+    qz = np.clip(az, 0, hw - rr)
+    exact = np.hypot(rho - qr, az - qz) - rr
+    got = np.array([SDF.lib.orc_sd_wheel(*p, *c, R, hw) for p in pts])
+    out = exact > 0
+    assert np.abs(got[out] - exact[out]).max() < 2e-6
+
+
+def test_material_groups_and_x_margin():
+    """Kind 16 scene: materials 7 and 8 map to groups 2 and 3, others to 1; cells with
+    x <= 1 or x >= nx - 2 are never solid (wgsl.ts:316)."""
+    P = orc.OPrim
+    prims = (P * 3)()
+    for k, (mat, x0) in enumerate([(0, 1.0), (7, 2.0), (8, 3.0)]):
+        prims[k].type, prims[k].material, prims[k].frame = 4, mat, 0
+        prims[k].a[0], prims[k].a[1], prims[k].a[2], prims[k].a[3] = x0, 0.5, 0.0, 0.3
+    prims2 = [prims[i] for i in range(3)]
+    extra = (P * 1)()
+    extra[0].type, extra[0].material, extra[0].frame = 4, 0, 0
+    extra[0].a[0], extra[0].a[1], extra[0].a[2], extra[0].a[3] = 0.0, 0.5, 0.0, 0.3
+    g = orc.grid((80, 24, 16), 0.05, (-0.2, 0.0, 0.0))
+    solid, _, _ = SDF.voxelize(g, shape(16), prims=prims2 + [extra[0]])
+    s3 = solid.reshape(16, 24, 80)
+    assert s3[0, 10, int((1.0 + 0.2) / 0.05)] == 1
+    assert s3[0, 10, int((2.0 + 0.2) / 0.05)] == 2
+    assert s3[0, 10, int((3.0 + 0.2) / 0.05)] == 3
+    assert not s3[:, :, :2].any() and not s3[:, :, -2:].any()
+    assert s3[:, :, 2].any()  # the sphere at x = 0 does reach x index 2
+
+
+def test_custom_profile_follows_circle():
+    R, cx0, cy0, cell = 0.35, 2.0, 0.8, 0.02
+    pw, ph = 256, 128
+    gx = 0.0 + (np.arange(pw) + 0.5) * cell
+    gy = 0.0 + (np.arange(ph) + 0.5) * cell
+    X, Y = np.meshgrid(gx, gy)
+    prof = (np.hypot(X - cx0, Y - cy0) - R).astype(np.float32).reshape(-1)
+    s = shape(4, width=0.3)
+    s.pScale = cell
+    g = orc.grid((100, 50, 20), 0.04, (0.0, 0.0, 0.0))
+    solid, _, _ = SDF.voxelize(g, s, profile=prof)
+    x, px, py, pz = centres(g)
+    d2 = np.hypot(px - cx0, py - cy0) - R
+    dz = np.abs(pz) - 0.3
+    exact = np.minimum(np.maximum(d2, dz), 0) + np.sqrt(np.maximum(d2, 0) ** 2 + np.maximum(dz, 0) ** 2)
+    thr = 0.3 * 0.04
+    band = np.abs(exact - thr) < 2e-3  # bilinear interpolation error of the 2 cm profile
+    want = (exact < thr) & (x > 1) & (x < 98)
+    assert np.array_equal((solid != 0)[~band], want[~band])
