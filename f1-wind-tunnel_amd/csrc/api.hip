@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "aero_internal.h"
+#include "sdf_f1.h"
 
 using namespace aero;
 
@@ -209,9 +210,9 @@ int aero_voxelize(aero_ctx* c, const aero_shape* s, const float* profile, const 
     CTXCHK(c);
     if (!s) return fail(c, AERO_E_INVALID, "aero_voxelize: null shape");
     const uint32_t kind = s->kind;
-    if (kind == AERO_KIND_F1 || kind == AERO_KIND_W17)
-        return fail(c, AERO_E_INVALID, "aero_voxelize: shape kind not available in this build");
-    const bool custom = !(kind == AERO_KIND_SPHERE || kind == AERO_KIND_CUBE || kind == AERO_KIND_WING ||
+    if (kind == AERO_KIND_W17 && !c->has_car)
+        return fail(c, AERO_E_STATE, "aero_voxelize: W17 kind needs aero_set_car_model first");
+    const bool custom = !(kind == AERO_KIND_F1 || kind == AERO_KIND_W17 || kind == AERO_KIND_SPHERE || kind == AERO_KIND_CUBE || kind == AERO_KIND_WING ||
                           kind == AERO_KIND_EMPTY || kind == AERO_KIND_SCENE);
     if (custom) {
         if (!profile) return fail(c, AERO_E_INVALID, "aero_voxelize: custom kind needs a profile");
@@ -223,13 +224,16 @@ int aero_voxelize(aero_ctx* c, const aero_shape* s, const float* profile, const 
         return fail(c, AERO_E_INVALID, "aero_voxelize: scene needs 1..4096 primitives");
 
     // Shape-only angles, evaluated once on the host (f1.ts:37, wgsl.ts:68,106).
-    float extra[4];
+    float extra[6];
     {
         const float th = atanf((s->rear_ride - s->ride_h) / (4.4f - 1.0f));
         extra[0] = cosf(th);
         extra[1] = sinf(th);
         extra[2] = cosf(-s->aoa);
         extra[3] = sinf(-s->aoa);
+        const float angm = f1::rear_wing_cfg(*s).angM;
+        extra[4] = cosf(angm);
+        extra[5] = sinf(angm);
     }
     float* d_profile = nullptr;
     aero_prim* d_prims = nullptr;

@@ -5,6 +5,7 @@
 // into cos/sin on the host once per call.
 #include "aero_internal.h"
 #include "sdf.h"
+#include "sdf_f1.h"
 
 namespace aero {
 
@@ -19,6 +20,7 @@ struct VoxArgs {
     float ox, oy, oz, h;
     float rake_c, rake_s;   // cos/sin of the rake angle (f1.ts:37)
     float wing_c, wing_s;   // cos/sin(-aoa) for kind 3
+    float rw_cos, rw_sin;   // cos/sin of the rear main-plane angle (kind 0)
     uint32_t* solid;
     float* emit;
 };
@@ -91,6 +93,12 @@ __device__ Res scene_sdf(const VoxArgs& a, v3 pw, float* emission) {
 __device__ Res scene(const VoxArgs& a, v3 p, float* emission) {
     *emission = 0.0f;
     switch (a.shape.kind) {
+    case AERO_KIND_F1: {
+        const f1::F1Ctx c{&a.shape, a.rake_c, a.rake_s, a.rw_cos, a.rw_sin, 0.0f};
+        const f1::R2 r = f1::sdF1(c, p);
+        *emission = f1::f1Emission(c, p);
+        return Res{r.d, r.m};
+    }
     case AERO_KIND_SPHERE: return Res{length(p - mk3(2.6f, 1.0f, 0.0f)) - 0.6f, 6.0f};
     case AERO_KIND_CUBE: return Res{sdRoundBox(p - mk3(2.6f, 1.0f, 0.0f), mk3(0.5f, 0.5f, 0.5f), 0.03f), 6.0f};
     case AERO_KIND_WING:
@@ -129,6 +137,7 @@ hipError_t launch_voxelize(aero_ctx* c, const aero_shape* s, const float* d_prof
     a.ox = c->grid.ox; a.oy = c->grid.oy; a.oz = c->grid.oz; a.h = c->grid.h;
     a.rake_c = host_extra[0]; a.rake_s = host_extra[1];
     a.wing_c = host_extra[2]; a.wing_s = host_extra[3];
+    a.rw_cos = host_extra[4]; a.rw_sin = host_extra[5];
     a.solid = c->solid;
     a.emit = c->emit;
     const dim3 grid((c->grid.nx + 63) / 64, (c->grid.ny + 3) / 4, c->grid.nz);

@@ -61,6 +61,11 @@ static const REAL WT[19] = {R(W0), R(W1), R(W1), R(W1), R(W1), R(W1), R(W1),
 
 static REAL dot3(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
+/* Square root in REAL: libm's sqrt returns double, which would silently lift the
+ * surrounding f32 expression to double.  sqrt of a float computed in double and
+ * rounded once to float is the correctly rounded f32 square root. */
+static REAL sqrt_r(REAL x) { return (REAL)sqrt((double)x); }
+
 /* wgsl.ts:135-138 */
 static REAL feq(uint32_t i, REAL rho, v3 u) {
     REAL cu = (REAL)CX[i] * u.x + (REAL)CY[i] * u.y + (REAL)CZ[i] * u.z;
@@ -187,7 +192,7 @@ static void step_cell(const og_grid* g, const og_sim* sim, const REAL* fin, REAL
     }
     REAL den = rho > R(1e-4) ? rho : R(1e-4); /* max(rho, 1e-4) */
     v3 u = {mom.x / den, mom.y / den, mom.z / den};
-    REAL ulen = sqrt(dot3(u, u));
+    REAL ulen = sqrt_r(dot3(u, u));
     if (is_bad(rho) || is_bad(u.x) || is_bad(u.y) || is_bad(u.z) || rho < R(0.3) || rho > R(3.0) ||
         ulen > R(0.35)) {
         rho = R(1.0);
@@ -216,12 +221,12 @@ static void step_cell(const og_grid* g, const og_sim* sim, const REAL* fin, REAL
         pxx += ne * cx * cx; pyy += ne * cy * cy; pzz += ne * cz * cz;
         pxy += ne * cx * cy; pxz += ne * cx * cz; pyz += ne * cy * cz;
     }
-    const REAL q = sqrt(pxx * pxx + pyy * pyy + pzz * pzz + R(2.0) * (pxy * pxy + pxz * pxz + pyz * pyz));
+    const REAL q = sqrt_r(pxx * pxx + pyy * pyy + pzz * pzz + R(2.0) * (pxy * pxy + pxz * pxz + pyz * pyz));
     const REAL fx = (REAL)x, fnx = (REAL)NX;
     const REAL tau0 = (REAL)sim->tau0 + R(0.4) * smoothstep_r(fnx - R(18.0), fnx - R(1.0), fx) +
                       R(0.25) * (R(1.0) - smoothstep_r(R(0.0), R(10.0), fx));
     const REAL smag = (REAL)sim->smag;
-    const REAL tau = R(0.5) * (tau0 + sqrt(tau0 * tau0 + R(18.0 * 1.41421356) * smag * smag * q / rho));
+    const REAL tau = R(0.5) * (tau0 + sqrt_r(tau0 * tau0 + R(18.0 * 1.41421356) * smag * smag * q / rho));
     const REAL omega = R(1.0) / tau;
     const REAL third = R(1.0 / 3.0);
     for (uint32_t i = 0; i < 19; i++) {
