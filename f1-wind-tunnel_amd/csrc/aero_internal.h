@@ -56,6 +56,8 @@ struct StepArgs {
     TileSlot* slots;
     const float* tau_x;     // tau0 + outlet/inlet sponge per x (wgsl.ts:259-260)
     uint64_t ns;            // population plane stride (elements)
+    uint32_t plane_bytes;   // ns * element size
+    uint32_t pop_bytes;     // 19 planes (below 4 GiB: buffer addressing)
     uint32_t nx, ny, nz;
     uint32_t tiles_x, tiles_y, ntiles;
     float u_in;

@@ -121,6 +121,10 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
     if (e != hipSuccess) { free_ctx(c); set_global_error(std::string("hipGetDevice: ") + hipGetErrorString(e)); return AERO_E_HIP; }
     c->ns = (n + 255) & ~(uint64_t)255;
     c->pop_bytes = (size_t)19 * c->ns * store_bytes(storage);
+    if (c->pop_bytes > 0xffffffffull) {
+        delete c;
+        return fail(nullptr, AERO_E_INVALID, "aero_create: a population array must stay below 4 GiB");
+    }
     c->tiles_x = (grid->nx + kTileX - 1) / kTileX;
     c->tiles_y = (grid->ny + kTileY - 1) / kTileY;
     c->ntiles = c->tiles_x * c->tiles_y * grid->nz;
@@ -137,7 +141,7 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
         {(void**)&c->temps[0], n * 4},
         {(void**)&c->temps[1], n * 4},
         {(void**)&c->tau_x, (size_t)grid->nx * 4},
-        {(void**)&c->slots, (size_t)c->ntiles * sizeof(TileSlot)},
+        {(void**)&c->slots, (size_t)c->ntiles * kTileY * sizeof(TileSlot)},
         {&c->red_out, sizeof(TileSlot)},
     };
     for (auto& a : allocs) {
@@ -159,7 +163,7 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
     e = hipMemsetAsync(c->solid, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->emit, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->flags, 0, c->ns, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * sizeof(TileSlot), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * kTileY * sizeof(TileSlot), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->temps[0], 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->temps[1], 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -198,7 +202,7 @@ int aero_init(aero_ctx* c) {
     const uint64_t n = c->grid.n;
     HIPCHK(c, hipMemsetAsync(c->temps[0], 0, n * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->temps[1], 0, n * 4, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * sizeof(TileSlot), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * kTileY * sizeof(TileSlot), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->cur = 0;
     c->tcur = 0;
@@ -351,7 +355,7 @@ int aero_read_forces(aero_ctx* c, double sum_fp64[9], int64_t sum_q[9], uint64_t
     HIPCHK(c, launch_reduce_slots(c));
     TileSlot r;
     HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(TileSlot), hipMemcpyDeviceToHost, c->stream));
-    if (reset) HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * sizeof(TileSlot), c->stream));
+    if (reset) HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * kTileY * sizeof(TileSlot), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (sum_fp64) for (int k = 0; k < 9; k++) sum_fp64[k] = r.d[k];
     if (sum_q) for (int k = 0; k < 9; k++) sum_q[k] = (int64_t)r.q[k];

@@ -25,6 +25,16 @@
 namespace aero {
 
 // ---- population storage formats ----
+// The stream-collide kernel moves populations with raw buffer instructions: one
+// resource per population array, the plane offset in an SGPR (soffset) and the
+// cell's byte offset in a VGPR, shared by every plane.  Stores are non-temporal.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr int kRawBufferWord3 = 0x00020000;  // raw (untyped) buffer, gfx9-family encoding
+constexpr int kAuxNT = 2;                    // non-temporal cache policy
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, kRawBufferWord3);
+}
+
 template <int S>
 struct Store;
 template <>
@@ -33,6 +43,13 @@ struct Store<AERO_STORE_F32> {
     static __device__ __forceinline__ float ld(const T* p, float) { return *p; }
     static __device__ __forceinline__ void st(T* p, float v, float) { __builtin_nontemporal_store(v, p); }
     static __host__ __device__ __forceinline__ T enc(float v, float) { return v; }
+    // The _b32 builtins move raw bits (unsigned int): bit-cast, never value-convert.
+    static __device__ __forceinline__ float bld(rsrc_t r, uint32_t voff, uint32_t soff, float) {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+    }
+    static __device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, uint32_t soff, float v, float) {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, kAuxNT);
+    }
 };
 template <>
 struct Store<AERO_STORE_F16S> {
@@ -42,6 +59,14 @@ struct Store<AERO_STORE_F16S> {
         __builtin_nontemporal_store((_Float16)(v - w), p);
     }
     static __host__ __device__ __forceinline__ T enc(float v, float w) { return (_Float16)(v - w); }
+    static __device__ __forceinline__ float bld(rsrc_t r, uint32_t voff, uint32_t soff, float w) {
+        const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+        return (float)__builtin_bit_cast(_Float16, b) + w;
+    }
+    static __device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, uint32_t soff, float v, float w) {
+        const _Float16 h = (_Float16)(v - w);
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), r, (int)voff, (int)soff, kAuxNT);
+    }
 };
 __host__ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
     uint32_t u = __builtin_bit_cast(uint32_t, x);
@@ -58,6 +83,13 @@ struct Store<AERO_STORE_BF16S> {
         __builtin_nontemporal_store(f32_to_bf16_rne(v - w), p);
     }
     static __host__ __device__ __forceinline__ T enc(float v, float w) { return f32_to_bf16_rne(v - w); }
+    static __device__ __forceinline__ float bld(rsrc_t r, uint32_t voff, uint32_t soff, float w) {
+        const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0);
+        return __builtin_bit_cast(float, (uint32_t)b << 16) + w;
+    }
+    static __device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, uint32_t soff, float v, float w) {
+        __builtin_amdgcn_raw_buffer_store_b16(f32_to_bf16_rne(v - w), r, (int)voff, (int)soff, kAuxNT);
+    }
 };
 
 // Workgroups are dealt round-robin to the 8 XCDs; map them so that XCD k receives
@@ -100,12 +132,26 @@ __device__ __forceinline__ void for19(F&& f) {
     unroll_seq(f, std::make_integer_sequence<int, 19>{});
 }
 
+// Element i of a wave-uniform base pointer, addressed with a 32-bit byte offset so
+// that the access compiles to an SGPR base plus one VGPR offset (population planes
+// are below 4 GiB).
+template <class T>
+__device__ __forceinline__ const T* at(const T* base, uint32_t i) {
+    return (const T*)((const char*)base + i * (uint32_t)sizeof(T));
+}
+template <class T>
+__device__ __forceinline__ T* at(T* base, uint32_t i) {
+    return (T*)((char*)base + i * (uint32_t)sizeof(T));
+}
+
 template <int S>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     using ST = Store<S>;
     using T = typename ST::T;
-    const T* __restrict__ fin = (const T*)a.fin;
-    T* __restrict__ fout = (T*)a.fout;
+    constexpr uint32_t ESZ = (uint32_t)sizeof(T);
+    const rsrc_t rin = make_rsrc(a.fin, a.pop_bytes);
+    const rsrc_t rout = make_rsrc(a.fout, a.pop_bytes);
+    const uint32_t pbytes = a.plane_bytes;  // bytes per population plane
 
     const uint32_t l = xcd_swizzle(blockIdx.x, gridDim.x);
     const uint32_t txy = a.tiles_x * a.tiles_y;
@@ -119,23 +165,31 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     const int y = (int)ty * kTileY + wv;  // wave-uniform
     const int z = (int)tz;                // block-uniform
     const int NX = (int)a.nx, NY = (int)a.ny, NZ = (int)a.nz;
-    const uint64_t ns = a.ns;
     const bool active = (x < NX) && (y < NY);
+    // Cell indices stay 32-bit (fewer than 2^31 cells); each population plane is a
+    // wave-uniform base pointer, so loads and stores use an SGPR base plus one
+    // 32-bit VGPR offset.
+    const uint32_t idx = (uint32_t)x + (uint32_t)NX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
+    const uint32_t fl = active ? (uint32_t)*at(a.flags, idx) : 0u;
+    // Fast path: the row is away from the floor, roof, symmetry plane and side, and no
+    // cell of this wavefront is solid or touches a solid.  Then every pull is a plain
+    // load (inlet/outlet lanes are patched with selects).
+    const bool yz_inner = (y >= 1) && (y <= NY - 2) && (z >= 1) && (z <= NZ - 2);
+    const bool fast = yz_inner && (__builtin_amdgcn_ballot_w64(active && (fl & (FL_GROUP | FL_NEAR)) != 0u) == 0);
+    const bool first_tile = (tx == 0), last_tile = (tx == a.tiles_x - 1);
 
     // Per-cell momentum-exchange force, groups rest/front/rear x (x,y,z).
     float g0x = 0.f, g0y = 0.f, g0z = 0.f, g1x = 0.f, g1y = 0.f, g1z = 0.f, g2x = 0.f, g2y = 0.f, g2z = 0.f;
     bool guard = false;
 
     if (active) {
-        const uint64_t idx = (uint64_t)x + (uint64_t)NX * ((uint64_t)y + (uint64_t)NY * (uint64_t)z);
-        const uint8_t fl = a.flags[idx];
         const bool outlet = (x == NX - 1);
-        const uint64_t pidx = ((uint64_t)y + (uint64_t)NY * (uint64_t)z) * 3u;
+        const uint32_t pidx = ((uint32_t)y + (uint32_t)NY * (uint32_t)z) * 3u;
         if (fl & FL_GROUP) {
             // Solid cell: fout_i = w_i, fields = 0 (wgsl.ts:182-186).
             for19([&](auto ic) {
                 constexpr int I = decltype(ic)::value;
-                ST::st(fout + (uint64_t)I * ns + idx, wt(I), wt(I));
+                ST::bst(rout, idx * ESZ, (uint32_t)I * pbytes, wt(I), wt(I));
             });
             if (a.write_fields) a.fields[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (outlet) { a.prev_u[pidx] = 0.f; a.prev_u[pidx + 1] = 0.f; a.prev_u[pidx + 2] = 0.f; }
@@ -144,73 +198,99 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
             const bool duct = (fl & FL_DUCT) != 0;
             const bool floor0 = (fl & FL_FLOOR0) != 0;
             float pux = 0.f, puy = 0.f, puz = 0.f, puu15 = 0.f;
-            if (outlet) {
+            if (last_tile && outlet) {
                 pux = a.prev_u[pidx]; puy = a.prev_u[pidx + 1]; puz = a.prev_u[pidx + 2];
                 puu15 = 1.5f * dot3(pux, puy, puz);
             }
-            const int64_t sxy = (int64_t)NX * (int64_t)NY;
+            const int32_t sxy = NX * NY;
 
             float f[19];
-            // Pull streaming with the boundary rules of wgsl.ts:203-229, tested in the
-            // reference's order: inlet, outlet, moving floor, far field, symmetry
-            // mirror, solid bounce-back, interior.
-            for19([&](auto ic) {
-                constexpr int I = decltype(ic)::value;
-                constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
-                constexpr float w = wt(I);
-                float v = 0.0f;
-                if constexpr (I == 0) {
-                    v = ST::ld(fin + idx, w);
-                } else {
-                    bool done = false;
-                    if constexpr (cx == 1) { if (x == 0) { v = a.feq_in[I]; done = true; } }
-                    if constexpr (cx == -1) {
-                        if (!done && outlet) { v = feq_c<I>(w, pux, puy, puz, puu15); done = true; }
-                    }
-                    if constexpr (cy == 1) {
-                        if (!done && y == 0) {
-                            v = ST::ld(fin + (uint64_t)kOPP[I] * ns + idx, wt(kOPP[I]));
-                            if constexpr (cx != 0) { if (!floor0) v = v + a.floor_term[I]; }
-                            done = true;
+            if (fast) {
+                // All 19 loads are issued before any lane-dependent patching, so no
+                // branch has to wait on a load in flight.
+                for19([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                    const int32_t off = cx + cy * NX + cz * sxy;
+                    f[I] = ST::bld(rin, (uint32_t)((int32_t)idx - off) * ESZ, (uint32_t)I * pbytes, wt(I));
+                });
+                if (first_tile) {  // inlet lane x = 0 (wgsl.ts:205)
+                    for19([&](auto ic) {
+                        constexpr int I = decltype(ic)::value;
+                        if constexpr (kCX[I] == 1) f[I] = (x == 0) ? a.feq_in[I] : f[I];
+                    });
+                }
+                if (last_tile) {  // outlet lane x = nx-1 (wgsl.ts:207)
+                    for19([&](auto ic) {
+                        constexpr int I = decltype(ic)::value;
+                        if constexpr (kCX[I] == -1) {
+                            const float e = feq_c<I>(wt(I), pux, puy, puz, puu15);
+                            f[I] = outlet ? e : f[I];
                         }
-                    }
-                    if constexpr (cy == -1) { if (!done && y == NY - 1) { v = a.feq_in[I]; done = true; } }
-                    if constexpr (cz == -1) { if (!done && z == NZ - 1) { v = a.feq_in[I]; done = true; } }
-                    if (!done) {
-                        bool mir = false;
-                        if constexpr (cz == 1) { mir = (z == 0); }
-                        const int64_t off = (int64_t)cx + (int64_t)cy * NX + (mir ? 0 : (int64_t)cz * sxy);
-                        const uint64_t sidx = (uint64_t)((int64_t)idx - off);
-                        uint32_t sg = 0;
-                        if (near) sg = a.flags[sidx] & FL_GROUP;
-                        if (sg != 0) {
-                            v = ST::ld(fin + (uint64_t)kOPP[I] * ns + idx, wt(kOPP[I]));
-                            if (!mir && !duct) {
-                                const float m = 2.0f * (v - w);
-                                // gforce[sg-1] -= m * c  (wgsl.ts:223)
+                    });
+                }
+            } else {
+                // Pull streaming with the boundary rules of wgsl.ts:203-229, tested in the
+                // reference's order: inlet, outlet, moving floor, far field, symmetry
+                // mirror, solid bounce-back, interior.
+                for19([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                    constexpr float w = wt(I);
+                    float v = 0.0f;
+                    if constexpr (I == 0) {
+                        v = ST::bld(rin, idx * ESZ, 0u, w);
+                    } else {
+                        bool done = false;
+                        if constexpr (cx == 1) { if (x == 0) { v = a.feq_in[I]; done = true; } }
+                        if constexpr (cx == -1) {
+                            if (!done && outlet) { v = feq_c<I>(w, pux, puy, puz, puu15); done = true; }
+                        }
+                        if constexpr (cy == 1) {
+                            if (!done && y == 0) {
+                                v = ST::bld(rin, idx * ESZ, (uint32_t)kOPP[I] * pbytes, wt(kOPP[I]));
+                                if constexpr (cx != 0) { if (!floor0) v = v + a.floor_term[I]; }
+                                done = true;
+                            }
+                        }
+                        if constexpr (cy == -1) { if (!done && y == NY - 1) { v = a.feq_in[I]; done = true; } }
+                        if constexpr (cz == -1) { if (!done && z == NZ - 1) { v = a.feq_in[I]; done = true; } }
+                        if (!done) {
+                            bool mir = false;
+                            if constexpr (cz == 1) { mir = (z == 0); }
+                            const int32_t off = cx + cy * NX + (mir ? 0 : cz * sxy);
+                            const uint32_t sidx = (uint32_t)((int32_t)idx - off);
+                            uint32_t sg = 0;
+                            if (near) sg = *at(a.flags, sidx) & FL_GROUP;
+                            if (sg != 0) {
+                                v = ST::bld(rin, idx * ESZ, (uint32_t)kOPP[I] * pbytes, wt(kOPP[I]));
+                                if (!mir && !duct) {
+                                    const float m = 2.0f * (v - w);
+                                    // gforce[sg-1] -= m * c  (wgsl.ts:223)
 #define AERO_ACC(G, gx, gy, gz)                                   \
     if (sg == G) {                                                \
         if constexpr (cx != 0) gx = (cx > 0) ? gx - m : gx + m;   \
         if constexpr (cy != 0) gy = (cy > 0) ? gy - m : gy + m;   \
         if constexpr (cz != 0) gz = (cz > 0) ? gz - m : gz + m;   \
     }
-                                AERO_ACC(1u, g0x, g0y, g0z)
-                                AERO_ACC(2u, g1x, g1y, g1z)
-                                AERO_ACC(3u, g2x, g2y, g2z)
+                                    AERO_ACC(1u, g0x, g0y, g0z)
+                                    AERO_ACC(2u, g1x, g1y, g1z)
+                                    AERO_ACC(3u, g2x, g2y, g2z)
 #undef AERO_ACC
-                            }
-                        } else {
-                            if constexpr (cz == 1) {
-                                if (mir) v = ST::ld(fin + (uint64_t)kZMIR[I] * ns + sidx, wt(kZMIR[I]));
-                                else v = ST::ld(fin + (uint64_t)I * ns + sidx, w);
+                                }
                             } else {
-                                v = ST::ld(fin + (uint64_t)I * ns + sidx, w);
+                                if constexpr (cz == 1) {
+                                    if (mir) v = ST::bld(rin, sidx * ESZ, (uint32_t)kZMIR[I] * pbytes, wt(kZMIR[I]));
+                                    else v = ST::bld(rin, sidx * ESZ, (uint32_t)I * pbytes, w);
+                                } else {
+                                    v = ST::bld(rin, sidx * ESZ, (uint32_t)I * pbytes, w);
+                                }
                             }
                         }
                     }
-                }
-                f[I] = v;
-            });
+                    f[I] = v;
+                });
+            }
 
             // Moments (wgsl.ts:231-236).
             float rho = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
@@ -267,7 +347,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
             // Smagorinsky relaxation time with the inlet/outlet sponge (wgsl.ts:258-262).
             const float q = __builtin_sqrtf(((pxx * pxx + pyy * pyy) + pzz * pzz) +
                                             2.0f * ((pxy * pxy + pxz * pxz) + pyz * pyz));
-            const float tau0 = a.tau_x[x];
+            const float tau0 = *at(a.tau_x, (uint32_t)x);
             const float tau = 0.5f * (tau0 + __builtin_sqrtf(tau0 * tau0 + a.smag_c * q / rho));
             const float omega = 1.0f / tau;
             const float om45 = (1.0f - omega) * 4.5f;
@@ -282,20 +362,17 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
                 if constexpr (cx * cz != 0) qp = qp + 2.0f * (cx * cz > 0 ? pxz : -pxz);
                 if constexpr (cy * cz != 0) qp = qp + 2.0f * (cy * cz > 0 ? pyz : -pyz);
                 const float v = eq[I] + (om45 * wt(I)) * qp;
-                ST::st(fout + (uint64_t)I * ns + idx, v, wt(I));
+                ST::bst(rout, idx * ESZ, (uint32_t)I * pbytes, v, wt(I));
             });
             if (a.write_fields) a.fields[idx] = make_float4(ux, uy, uz, rho);
-            if (outlet) { a.prev_u[pidx] = ux; a.prev_u[pidx + 1] = uy; a.prev_u[pidx + 2] = uz; }
+            if (last_tile && outlet) { a.prev_u[pidx] = ux; a.prev_u[pidx + 1] = uy; a.prev_u[pidx + 2] = uz; }
         }
     }
 
-    // ---- force and guard reduction into the tile's slot ----
+    // ---- force and guard reduction into the wavefront's own slot (no block barrier) ----
     const bool has = guard || g0x != 0.f || g0y != 0.f || g0z != 0.f || g1x != 0.f || g1y != 0.f ||
                      g1z != 0.f || g2x != 0.f || g2y != 0.f || g2z != 0.f;
-    if (__syncthreads_or(has)) {
-        __shared__ double sd[4][9];
-        __shared__ long long sq[4][6];
-        __shared__ unsigned int sgd[4];
+    if (__builtin_amdgcn_ballot_w64(has) != 0) {
         double d[9] = {g0x, g0y, g0z, g1x, g1y, g1z, g2x, g2y, g2z};
         // wgsl.ts:280-282: i32(round(f * 1e4)) when non-zero; WGSL round is half-to-even.
         const float gq[6] = {g0x, g0y, g1x, g1y, g2x, g2y};
@@ -312,21 +389,13 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
             gc += __shfl_xor(gc, o, 64);
         }
         if (lane == 0) {
+            TileSlot* s = a.slots + ((size_t)l * kTileY + wv);
 #pragma unroll
-            for (int k = 0; k < 9; k++) sd[wv][k] = d[k];
-#pragma unroll
-            for (int k = 0; k < 6; k++) sq[wv][k] = qv[k];
-            sgd[wv] = gc;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            TileSlot* s = a.slots + l;
-#pragma unroll
-            for (int k = 0; k < 9; k++) s->d[k] += ((sd[0][k] + sd[1][k]) + sd[2][k]) + sd[3][k];
+            for (int k = 0; k < 9; k++) s->d[k] += d[k];
             const int qmap[6] = {0, 1, 3, 4, 6, 7};
 #pragma unroll
-            for (int k = 0; k < 6; k++) s->q[qmap[k]] += ((sq[0][k] + sq[1][k]) + sq[2][k]) + sq[3][k];
-            s->guard += (unsigned long long)sgd[0] + sgd[1] + sgd[2] + sgd[3];
+            for (int k = 0; k < 6; k++) s->q[qmap[k]] += qv[k];
+            s->guard += gc;
         }
     }
 }
@@ -461,6 +530,8 @@ static float feq_host(int i, float rho, float ux, float uy, float uz) {
 
 void fill_step_consts(const aero_ctx* c, StepArgs* a) {
     a->ns = c->ns;
+    a->plane_bytes = (uint32_t)(c->ns * (c->storage == AERO_STORE_F32 ? 4u : 2u));
+    a->pop_bytes = (uint32_t)c->pop_bytes;
     a->nx = c->grid.nx; a->ny = c->grid.ny; a->nz = c->grid.nz;
     a->tiles_x = c->tiles_x; a->tiles_y = c->tiles_y; a->ntiles = c->ntiles;
     a->u_in = c->sim.u_in;
@@ -514,7 +585,7 @@ hipError_t launch_step(aero_ctx* c, int write_fields) {
 }
 
 hipError_t launch_reduce_slots(aero_ctx* c) {
-    hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->slots, c->ntiles, (TileSlot*)c->red_out);
+    hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->slots, c->ntiles * (uint32_t)kTileY, (TileSlot*)c->red_out);
     return hipGetLastError();
 }
 
