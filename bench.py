@@ -1,7 +1,7 @@
 """Benchmark of the wind-tunnel hot path on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload f1|cube|large]
-                    [--storage f32|f16s|bf16s] [--no-cpu]
+                    [--storage f32|f16s|bf16s] [--math fast|exact] [--no-cpu]
 
 Workload (default "f1"): the generic F1 body, 320x96x96 half tunnel at 4.5 cm
 cells, FP32 populations, D3Q19 regularized + Smagorinsky, moving floor, sponge
@@ -74,6 +74,7 @@ def main() -> None:
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="f1", choices=["f1", "cube", "large"])
     ap.add_argument("--storage", default="f32", choices=["f32", "f16s", "bf16s"])
+    ap.add_argument("--math", default="fast", choices=["fast", "exact"])
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -87,7 +88,7 @@ def main() -> None:
 
     wl = {"f1": W.f1, "cube": W.cube, "large": W.large}[args.workload]()
     print(f"[bench] workload {wl.name} {wl.dims} storage {args.storage}: voxelize + init", file=sys.stderr, flush=True)
-    e = aerolbm.HipAeroEngine(storage=args.storage, **wl.engine_kwargs())
+    e = aerolbm.HipAeroEngine(storage=args.storage, math=args.math, **wl.engine_kwargs())
     e.set_shape(wl.shape, prims=wl.prims)
     e.reset()
 
@@ -136,9 +137,10 @@ def main() -> None:
         "config": {
             "workload": f"{wl.name}: {wl.dims[0]}x{wl.dims[1]}x{wl.dims[2]} half tunnel, h={wl.h:.4g} m, "
                         f"D3Q19 regularized+Smagorinsky Cs={wl.smag}, u={wl.u_lb}, tau0={wl.tau0}, "
-                        f"{args.storage} populations, heat 2 per 8 steps",
+                        f"{args.storage} populations, {args.math} collision, heat 2 per 8 steps",
             "cells": wl.cells,
             "storage": args.storage,
+            "math": args.math,
         },
         "roofline": {
             "bound": "hbm",
@@ -147,7 +149,7 @@ def main() -> None:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None,
-            "kernel": f"k_step<{args.storage}>",
+            "kernel": f"k_step<{args.storage},{args.math}>",
             "bytes_per_cell": bpu,
             "launch_us": round(launch_ms * 1e3, 2),
         },

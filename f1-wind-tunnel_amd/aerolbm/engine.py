@@ -124,12 +124,15 @@ class HipAeroEngine:
     ``quality`` selects the reference presets (h = 10.4 m / nx, origin (-2.4, 0, 0));
     explicit ``dims``/``h``/``origin`` describe any other tunnel (the benchmark
     workloads need h = 1/22 m or 4.5 cm cells, which the preset rule cannot give).
+    ``math="exact"`` evaluates the collision in the reference's operation order
+    (bit-identical to an f32 evaluation of the WGSL); ``"fast"`` uses the moment-space
+    form with FMAs.
     """
 
     def __init__(self, quality: str = "medium", *, dims: Optional[Sequence[int]] = None,
                  h: Optional[float] = None, origin: Optional[Sequence[float]] = None,
                  u_lb: float = LATTICE_U, tau0: float = TAU0, smag: float = SMAG,
-                 storage: str = "f32", kmh: float = 250.0):
+                 storage: str = "f32", math: str = "fast", kmh: float = 250.0):
         nx, ny, nz = dims if dims is not None else QUALITIES[quality]
         self.dims = (int(nx), int(ny), int(nz))
         self.n = nx * ny * nz
@@ -140,6 +143,9 @@ class HipAeroEngine:
         self.smag = float(smag)
         self.kmh = float(kmh)
         self.storage = storage
+        if math not in ("fast", "exact"):
+            raise ValueError("math must be 'fast' or 'exact'")
+        self.math = math
         self.heat_params = Heat(3.0, HEAT_COOL, HEAT_DIFFUSE, 0.0)
         self.on_forces: Optional[Callable[[Forces], None]] = None
         self.frame_no = 0
@@ -148,8 +154,8 @@ class HipAeroEngine:
         self.grid = Grid(nx, ny, nz, self.n, self.origin[0], self.origin[1], self.origin[2], self.h)
         self.sim = Sim(self.u_lb, self.tau0, self.smag, 0.0)
         ctx = C.c_void_p()
-        check(self._lib.aero_create(C.byref(self.grid), C.byref(self.sim), _lib.STORAGE_NAMES[storage],
-                                    C.byref(ctx)), None, "aero_create")
+        code = _lib.STORAGE_NAMES[storage] | (_lib.MATH_EXACT if math == "exact" else 0)
+        check(self._lib.aero_create(C.byref(self.grid), C.byref(self.sim), code, C.byref(ctx)), None, "aero_create")
         self.ctx = ctx
         self.profile: Optional[CustomProfile] = None
 

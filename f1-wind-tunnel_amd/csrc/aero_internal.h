@@ -5,7 +5,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/aero_lbm.h"
+#include "aero_lbm.h"
 
 namespace aero {
 
@@ -31,11 +31,13 @@ enum : uint8_t {
     FL_HOTNB = 0x20,  // some in-range 6-neighbour has emit > 0 (heat source relaxation)
 };
 
-// Work tile of the stream-collide kernel: 64 cells along x (one wavefront) by 4
-// rows along y (four wavefronts) in one z plane.
+// Work tile of the stream-collide kernel: kTileX cells along x by kTileY rows in one
+// z plane, 256 threads; every wavefront lies inside one row (kTileX >= 64).
 constexpr int kTileX = 64;
 constexpr int kTileY = 4;
 constexpr int kBlock = kTileX * kTileY;
+constexpr int kWaves = kBlock / 64;  // force slots per tile (one per wavefront)
+static_assert(kBlock == 256 && kTileX % 64 == 0, "tile must be 256 threads, whole wavefronts per row");
 
 // Force/guard partial sums owned by one tile.  Each tile adds its own contribution
 // in place every step, so the totals are reproducible run to run; aero_read_forces
@@ -63,6 +65,7 @@ struct StepArgs {
     float u_in;
     float smag_c;           // f32((18*1.41421356)) * Cs * Cs  (wgsl.ts:261)
     float feq_in[19];       // feq(i, 1, (u_in, 0, 0))
+    float feq_in_dev[19];   // feq_in - w_i (deviation form)
     float floor_term[19];   // 6 * w_i * c_x,i * u_in  (moving floor, wgsl.ts:210)
     int write_fields;
 };
@@ -73,6 +76,7 @@ struct aero_ctx {
     aero_grid grid;
     aero_sim sim;
     int storage;
+    int exact;              // AERO_MATH_EXACT: reference operation order, bit-exact
     int device;
     hipStream_t stream;
     hipEvent_t ev0, ev1;

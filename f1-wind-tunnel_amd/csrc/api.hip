@@ -107,6 +107,8 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
     if (n != grid->n) return fail(nullptr, AERO_E_INVALID, "aero_create: grid.n != nx*ny*nz");
     if (n > (1ull << 31)) return fail(nullptr, AERO_E_INVALID, "aero_create: more than 2^31 cells");
     if (grid->nz > 65535u) return fail(nullptr, AERO_E_INVALID, "aero_create: nz above 65535");
+    const int exact = (storage & AERO_MATH_EXACT) ? 1 : 0;
+    storage &= ~AERO_MATH_EXACT;
     if (storage != AERO_STORE_F32 && storage != AERO_STORE_F16S && storage != AERO_STORE_BF16S)
         return fail(nullptr, AERO_E_INVALID, "aero_create: unknown storage format");
     if (!(grid->h > 0.0f) || !isfinite(grid->h)) return fail(nullptr, AERO_E_INVALID, "aero_create: bad cell size");
@@ -117,6 +119,7 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
     c->grid = *grid;
     c->sim = *sim;
     c->storage = storage;
+    c->exact = exact;
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) { free_ctx(c); set_global_error(std::string("hipGetDevice: ") + hipGetErrorString(e)); return AERO_E_HIP; }
     c->ns = (n + 255) & ~(uint64_t)255;
@@ -141,7 +144,7 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
         {(void**)&c->temps[0], n * 4},
         {(void**)&c->temps[1], n * 4},
         {(void**)&c->tau_x, (size_t)grid->nx * 4},
-        {(void**)&c->slots, (size_t)c->ntiles * kTileY * sizeof(TileSlot)},
+        {(void**)&c->slots, (size_t)c->ntiles * kWaves * sizeof(TileSlot)},
         {&c->red_out, sizeof(TileSlot)},
     };
     for (auto& a : allocs) {
@@ -163,7 +166,7 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
     e = hipMemsetAsync(c->solid, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->emit, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->flags, 0, c->ns, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * kTileY * sizeof(TileSlot), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * kWaves * sizeof(TileSlot), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->temps[0], 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->temps[1], 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -202,7 +205,7 @@ int aero_init(aero_ctx* c) {
     const uint64_t n = c->grid.n;
     HIPCHK(c, hipMemsetAsync(c->temps[0], 0, n * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->temps[1], 0, n * 4, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * kTileY * sizeof(TileSlot), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * kWaves * sizeof(TileSlot), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->cur = 0;
     c->tcur = 0;
@@ -355,7 +358,7 @@ int aero_read_forces(aero_ctx* c, double sum_fp64[9], int64_t sum_q[9], uint64_t
     HIPCHK(c, launch_reduce_slots(c));
     TileSlot r;
     HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(TileSlot), hipMemcpyDeviceToHost, c->stream));
-    if (reset) HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * kTileY * sizeof(TileSlot), c->stream));
+    if (reset) HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * kWaves * sizeof(TileSlot), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (sum_fp64) for (int k = 0; k < 9; k++) sum_fp64[k] = r.d[k];
     if (sum_q) for (int k = 0; k < 9; k++) sum_q[k] = (int64_t)r.q[k];

@@ -50,6 +50,8 @@ struct Store<AERO_STORE_F32> {
     static __device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, uint32_t soff, float v, float) {
         __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, kAuxNT);
     }
+    static __device__ __forceinline__ float braw(rsrc_t r, uint32_t voff, uint32_t soff) { return bld(r, voff, soff, 0.f); }
+    static __device__ __forceinline__ void bstraw(rsrc_t r, uint32_t voff, uint32_t soff, float v) { bst(r, voff, soff, v, 0.f); }
 };
 template <>
 struct Store<AERO_STORE_F16S> {
@@ -64,8 +66,13 @@ struct Store<AERO_STORE_F16S> {
         return (float)__builtin_bit_cast(_Float16, b) + w;
     }
     static __device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, uint32_t soff, float v, float w) {
-        const _Float16 h = (_Float16)(v - w);
-        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, h), r, (int)voff, (int)soff, kAuxNT);
+        bstraw(r, voff, soff, v - w);
+    }
+    static __device__ __forceinline__ float braw(rsrc_t r, uint32_t voff, uint32_t soff) {
+        return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0));
+    }
+    static __device__ __forceinline__ void bstraw(rsrc_t r, uint32_t voff, uint32_t soff, float d) {
+        __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (_Float16)d), r, (int)voff, (int)soff, kAuxNT);
     }
 };
 __host__ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
@@ -88,7 +95,13 @@ struct Store<AERO_STORE_BF16S> {
         return __builtin_bit_cast(float, (uint32_t)b << 16) + w;
     }
     static __device__ __forceinline__ void bst(rsrc_t r, uint32_t voff, uint32_t soff, float v, float w) {
-        __builtin_amdgcn_raw_buffer_store_b16(f32_to_bf16_rne(v - w), r, (int)voff, (int)soff, kAuxNT);
+        bstraw(r, voff, soff, v - w);
+    }
+    static __device__ __forceinline__ float braw(rsrc_t r, uint32_t voff, uint32_t soff) {
+        return __builtin_bit_cast(float, (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, (int)voff, (int)soff, 0) << 16);
+    }
+    static __device__ __forceinline__ void bstraw(rsrc_t r, uint32_t voff, uint32_t soff, float d) {
+        __builtin_amdgcn_raw_buffer_store_b16(f32_to_bf16_rne(d), r, (int)voff, (int)soff, kAuxNT);
     }
 };
 
@@ -144,10 +157,32 @@ __device__ __forceinline__ T* at(T* base, uint32_t i) {
     return (T*)((char*)base + i * (uint32_t)sizeof(T));
 }
 
-template <int S>
+// Feq deviation feq(I, 1, u) - w_I = w_I (3 c.u + 4.5 (c.u)^2 - 1.5 u.u)  (fast mode).
+template <int I>
+__device__ __forceinline__ float feq_dev(float ux, float uy, float uz, float uu15) {
+    constexpr float w = wt(I);
+    if constexpr (I == 0) {
+        return -w * uu15;
+    } else {
+        const float cu = cdotu<I>(ux, uy, uz);
+        return w * (__builtin_fmaf(4.5f * cu, cu, 3.0f * cu) - uu15);
+    }
+}
+
+// One fused stream-collide step.
+//   EXACT = true : the reference's per-direction arithmetic in its operation order
+//                  (wgsl.ts:231-271); bit-identical to the f32 restatement.
+//   EXACT = false: the same regularized collision evaluated in moment space with
+//                  FMAs: rho, j, the second moments and Pi^neq from 18 pair sums,
+//                  then every outgoing population from a handful of per-cell
+//                  scalars.  Shifted storage stays in deviation form (f - w)
+//                  throughout, which also keeps the small departures from
+//                  equilibrium out of the cancellation against w.
+template <int S, bool EXACT>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     using ST = Store<S>;
     using T = typename ST::T;
+    constexpr bool DEV = !EXACT && (S != AERO_STORE_F32);  // pulled values are f - w
     constexpr uint32_t ESZ = (uint32_t)sizeof(T);
     const rsrc_t rin = make_rsrc(a.fin, a.pop_bytes);
     const rsrc_t rout = make_rsrc(a.fout, a.pop_bytes);
@@ -161,22 +196,31 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     const uint32_t tx = rem - ty * a.tiles_x;
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const int x = (int)tx * kTileX + lane;
-    const int y = (int)ty * kTileY + wv;  // wave-uniform
+    const int x = (int)tx * kTileX + (int)(threadIdx.x % kTileX);
+    const int y = (int)ty * kTileY + (int)(threadIdx.x / kTileX);  // wave-uniform
     const int z = (int)tz;                // block-uniform
     const int NX = (int)a.nx, NY = (int)a.ny, NZ = (int)a.nz;
     const bool active = (x < NX) && (y < NY);
-    // Cell indices stay 32-bit (fewer than 2^31 cells); each population plane is a
-    // wave-uniform base pointer, so loads and stores use an SGPR base plus one
-    // 32-bit VGPR offset.
+    // Cell indices stay 32-bit (fewer than 2^31 cells).
     const uint32_t idx = (uint32_t)x + (uint32_t)NX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
-    const uint32_t fl = active ? (uint32_t)*at(a.flags, idx) : 0u;
-    // Fast path: the row is away from the floor, roof, symmetry plane and side, and no
-    // cell of this wavefront is solid or touches a solid.  Then every pull is a plain
-    // load (inlet/outlet lanes are patched with selects).
+    const int32_t sxy = NX * NY;
+    // Rows away from the floor, roof, symmetry plane and side ("inner" rows) whose
+    // wavefront holds no solid and no solid-touching cell pull every population with a
+    // plain shifted load (inlet/outlet lanes are patched afterwards).  Every other
+    // wavefront takes the general path below, which also issues its 19 loads together.
     const bool yz_inner = (y >= 1) && (y <= NY - 2) && (z >= 1) && (z <= NZ - 2);
-    const bool fast = yz_inner && (__builtin_amdgcn_ballot_w64(active && (fl & (FL_GROUP | FL_NEAR)) != 0u) == 0);
     const bool first_tile = (tx == 0), last_tile = (tx == a.tiles_x - 1);
+    const uint32_t fl = active ? (uint32_t)*at(a.flags, idx) : 0u;
+    const bool fast = yz_inner && (__builtin_amdgcn_ballot_w64(active && (fl & (FL_GROUP | FL_NEAR)) != 0u) == 0);
+
+    auto LD = [&](uint32_t voff, uint32_t soff, float w) -> float {
+        if constexpr (DEV) return ST::braw(rin, voff, soff);
+        else return ST::bld(rin, voff, soff, w);
+    };
+    auto SV = [&](uint32_t voff, uint32_t soff, float v, float w) {
+        if constexpr (DEV) ST::bstraw(rout, voff, soff, v);
+        else ST::bst(rout, voff, soff, v, w);
+    };
 
     // Per-cell momentum-exchange force, groups rest/front/rear x (x,y,z).
     float g0x = 0.f, g0y = 0.f, g0z = 0.f, g1x = 0.f, g1y = 0.f, g1z = 0.f, g2x = 0.f, g2y = 0.f, g2z = 0.f;
@@ -189,7 +233,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
             // Solid cell: fout_i = w_i, fields = 0 (wgsl.ts:182-186).
             for19([&](auto ic) {
                 constexpr int I = decltype(ic)::value;
-                ST::bst(rout, idx * ESZ, (uint32_t)I * pbytes, wt(I), wt(I));
+                SV(idx * ESZ, (uint32_t)I * pbytes, DEV ? 0.0f : wt(I), wt(I));
             });
             if (a.write_fields) a.fields[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (outlet) { a.prev_u[pidx] = 0.f; a.prev_u[pidx + 1] = 0.f; a.prev_u[pidx + 2] = 0.f; }
@@ -202,168 +246,324 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
                 pux = a.prev_u[pidx]; puy = a.prev_u[pidx + 1]; puz = a.prev_u[pidx + 2];
                 puu15 = 1.5f * dot3(pux, puy, puz);
             }
-            const int32_t sxy = NX * NY;
+            // value of the outlet pull feq(I, 1, prevU) in the working domain
+            auto outlet_val = [&](auto ic) -> float {
+                constexpr int I = decltype(ic)::value;
+                if constexpr (DEV) return feq_dev<I>(pux, puy, puz, puu15);
+                else return feq_c<I>(wt(I), pux, puy, puz, puu15);
+            };
 
             float f[19];
             if (fast) {
-                // All 19 loads are issued before any lane-dependent patching, so no
-                // branch has to wait on a load in flight.
                 for19([&](auto ic) {
                     constexpr int I = decltype(ic)::value;
-                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
-                    const int32_t off = cx + cy * NX + cz * sxy;
-                    f[I] = ST::bld(rin, (uint32_t)((int32_t)idx - off) * ESZ, (uint32_t)I * pbytes, wt(I));
+                    const int32_t off = kCX[I] + kCY[I] * NX + kCZ[I] * sxy;
+                    f[I] = LD((uint32_t)((int32_t)idx - off) * ESZ, (uint32_t)I * pbytes, wt(I));
                 });
                 if (first_tile) {  // inlet lane x = 0 (wgsl.ts:205)
                     for19([&](auto ic) {
                         constexpr int I = decltype(ic)::value;
-                        if constexpr (kCX[I] == 1) f[I] = (x == 0) ? a.feq_in[I] : f[I];
+                        if constexpr (kCX[I] == 1) f[I] = (x == 0) ? (DEV ? a.feq_in_dev[I] : a.feq_in[I]) : f[I];
                     });
                 }
                 if (last_tile) {  // outlet lane x = nx-1 (wgsl.ts:207)
                     for19([&](auto ic) {
                         constexpr int I = decltype(ic)::value;
                         if constexpr (kCX[I] == -1) {
-                            const float e = feq_c<I>(wt(I), pux, puy, puz, puu15);
+                            const float e = outlet_val(ic);
                             f[I] = outlet ? e : f[I];
                         }
                     });
                 }
             } else {
-                // Pull streaming with the boundary rules of wgsl.ts:203-229, tested in the
-                // reference's order: inlet, outlet, moving floor, far field, symmetry
-                // mirror, solid bounce-back, interior.
+                // General path: the boundary rules of wgsl.ts:203-229 in the reference's
+                // order (inlet, outlet, moving floor, far field, symmetry mirror, solid
+                // bounce-back, interior), evaluated without serialising the loads:
+                //   (1) solid-touching lanes fetch their 18 neighbour groups together;
+                //   (2) every direction gets a per-lane byte offset into the population
+                //       array (constant-valued pulls read the cell's own value);
+                //   (3) the 19 loads go out together, then constants, the floor term
+                //       and the momentum exchange are applied.
+                const bool flo_row = (y == 0), far_y = (y == NY - 1), far_z = (z == NZ - 1), mir_row = (z == 0);
+                uint32_t s1 = 0u, s2 = 0u, s3 = 0u;  // bit I: source of direction I is solid of group 1/2/3
+                auto special = [&](auto ic) -> bool {  // inlet/outlet/floor/far field: no solid test
+                    constexpr int I = decltype(ic)::value;
+                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                    bool sp = false;
+                    if constexpr (cx == 1) sp = sp || (x == 0);
+                    if constexpr (cx == -1) sp = sp || outlet;
+                    if constexpr (cy == 1) sp = sp || flo_row;
+                    if constexpr (cy == -1) sp = sp || far_y;
+                    if constexpr (cz == -1) sp = sp || far_z;
+                    return sp;
+                };
+                auto src_off = [&](auto ic) -> int32_t {  // element shift of the source cell
+                    constexpr int I = decltype(ic)::value;
+                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                    if constexpr (cz == 1) return mir_row ? cx + cy * NX : cx + cy * NX + sxy;
+                    else return cx + cy * NX + cz * sxy;
+                };
+                if (__builtin_amdgcn_ballot_w64(near) != 0) {
+                    uint32_t nb[19];
+                    for19([&](auto ic) {
+                        constexpr int I = decltype(ic)::value;
+                        if constexpr (I != 0) {
+                            const bool test = near && !special(ic);
+                            const uint32_t sidx = test ? (uint32_t)((int32_t)idx - src_off(ic)) : idx;
+                            nb[I] = *at(a.flags, sidx);
+                        }
+                    });
+                    for19([&](auto ic) {
+                        constexpr int I = decltype(ic)::value;
+                        if constexpr (I != 0) {
+                            const uint32_t g = (near && !special(ic)) ? (nb[I] & FL_GROUP) : 0u;
+                            s1 |= (g == 1u ? 1u : 0u) << I;
+                            s2 |= (g == 2u ? 1u : 0u) << I;
+                            s3 |= (g == 3u ? 1u : 0u) << I;
+                        }
+                    });
+                }
+                const uint32_t sany = s1 | s2 | s3;
+                const uint32_t own = idx * ESZ;
+                uint32_t voff[19];
+                for19([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                    if constexpr (I == 0) {
+                        voff[0] = own;
+                    } else {
+                        // plane of a regular or mirrored pull
+                        uint32_t plane = (uint32_t)I;
+                        if constexpr (cz == 1) plane = mir_row ? (uint32_t)kZMIR[I] : (uint32_t)I;
+                        const uint32_t reg = (uint32_t)((int32_t)idx - src_off(ic)) * ESZ + plane * pbytes;
+                        const uint32_t opp = own + (uint32_t)kOPP[I] * pbytes;
+                        const uint32_t dummy = own + (uint32_t)I * pbytes;
+                        bool constant = false, bounce = false;
+                        if constexpr (cx == 1) constant = constant || (x == 0);
+                        if constexpr (cx == -1) constant = constant || outlet;
+                        if constexpr (cy == 1) bounce = !constant && flo_row;
+                        bool farf = false;
+                        if constexpr (cy == -1) farf = farf || far_y;
+                        if constexpr (cz == -1) farf = farf || far_z;
+                        if constexpr (cy != 1) constant = constant || farf;
+                        else constant = constant || (farf && !flo_row);
+                        bounce = bounce || (((sany >> I) & 1u) != 0u);
+                        voff[I] = constant ? dummy : (bounce ? opp : reg);
+                    }
+                });
+                for19([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    f[I] = LD(voff[I], 0u, wt(kOPP[I]) /* w_opp == w_I */);
+                });
                 for19([&](auto ic) {
                     constexpr int I = decltype(ic)::value;
                     constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
                     constexpr float w = wt(I);
-                    float v = 0.0f;
-                    if constexpr (I == 0) {
-                        v = ST::bld(rin, idx * ESZ, 0u, w);
-                    } else {
+                    if constexpr (I != 0) {
+                        const float inl = DEV ? a.feq_in_dev[I] : a.feq_in[I];
                         bool done = false;
-                        if constexpr (cx == 1) { if (x == 0) { v = a.feq_in[I]; done = true; } }
-                        if constexpr (cx == -1) {
-                            if (!done && outlet) { v = feq_c<I>(w, pux, puy, puz, puu15); done = true; }
-                        }
+                        float v = f[I];
+                        if constexpr (cx == 1) { if (x == 0) { v = inl; done = true; } }
+                        if constexpr (cx == -1) { if (!done && outlet) { v = outlet_val(ic); done = true; } }
                         if constexpr (cy == 1) {
-                            if (!done && y == 0) {
-                                v = ST::bld(rin, idx * ESZ, (uint32_t)kOPP[I] * pbytes, wt(kOPP[I]));
+                            if (!done && flo_row) {  // moving floor: fin[opp] + 6 w c_x u_floor
                                 if constexpr (cx != 0) { if (!floor0) v = v + a.floor_term[I]; }
                                 done = true;
                             }
                         }
-                        if constexpr (cy == -1) { if (!done && y == NY - 1) { v = a.feq_in[I]; done = true; } }
-                        if constexpr (cz == -1) { if (!done && z == NZ - 1) { v = a.feq_in[I]; done = true; } }
-                        if (!done) {
+                        if constexpr (cy == -1) { if (!done && far_y) { v = inl; done = true; } }
+                        if constexpr (cz == -1) { if (!done && far_z) { v = inl; done = true; } }
+                        if (!done && ((sany >> I) & 1u) != 0u) {
                             bool mir = false;
-                            if constexpr (cz == 1) { mir = (z == 0); }
-                            const int32_t off = cx + cy * NX + (mir ? 0 : cz * sxy);
-                            const uint32_t sidx = (uint32_t)((int32_t)idx - off);
-                            uint32_t sg = 0;
-                            if (near) sg = *at(a.flags, sidx) & FL_GROUP;
-                            if (sg != 0) {
-                                v = ST::bld(rin, idx * ESZ, (uint32_t)kOPP[I] * pbytes, wt(kOPP[I]));
-                                if (!mir && !duct) {
-                                    const float m = 2.0f * (v - w);
-                                    // gforce[sg-1] -= m * c  (wgsl.ts:223)
-#define AERO_ACC(G, gx, gy, gz)                                   \
-    if (sg == G) {                                                \
-        if constexpr (cx != 0) gx = (cx > 0) ? gx - m : gx + m;   \
-        if constexpr (cy != 0) gy = (cy > 0) ? gy - m : gy + m;   \
-        if constexpr (cz != 0) gz = (cz > 0) ? gz - m : gz + m;   \
+                            if constexpr (cz == 1) mir = mir_row;
+                            if (!mir && !duct) {  // gforce[sg-1] -= 2 (v - w) c  (wgsl.ts:223)
+                                const float m = 2.0f * (DEV ? v : v - w);
+#define AERO_ACC(MASK, gx, gy, gz)                                  \
+    if (((MASK) >> I) & 1u) {                                       \
+        if constexpr (cx != 0) gx = (cx > 0) ? gx - m : gx + m;     \
+        if constexpr (cy != 0) gy = (cy > 0) ? gy - m : gy + m;     \
+        if constexpr (cz != 0) gz = (cz > 0) ? gz - m : gz + m;     \
     }
-                                    AERO_ACC(1u, g0x, g0y, g0z)
-                                    AERO_ACC(2u, g1x, g1y, g1z)
-                                    AERO_ACC(3u, g2x, g2y, g2z)
+                                AERO_ACC(s1, g0x, g0y, g0z)
+                                AERO_ACC(s2, g1x, g1y, g1z)
+                                AERO_ACC(s3, g2x, g2y, g2z)
 #undef AERO_ACC
-                                }
-                            } else {
-                                if constexpr (cz == 1) {
-                                    if (mir) v = ST::bld(rin, sidx * ESZ, (uint32_t)kZMIR[I] * pbytes, wt(kZMIR[I]));
-                                    else v = ST::bld(rin, sidx * ESZ, (uint32_t)I * pbytes, w);
-                                } else {
-                                    v = ST::bld(rin, sidx * ESZ, (uint32_t)I * pbytes, w);
-                                }
                             }
                         }
+                        f[I] = v;
                     }
-                    f[I] = v;
                 });
             }
 
-            // Moments (wgsl.ts:231-236).
-            float rho = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
-            bool fx = true, fy = true, fz = true;
-            for19([&](auto ic) {
-                constexpr int I = decltype(ic)::value;
-                constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
-                rho = (I == 0) ? f[0] : rho + f[I];
-                if constexpr (cx != 0) { float t = cx > 0 ? f[I] : -f[I]; mx = fx ? t : mx + t; fx = false; }
-                if constexpr (cy != 0) { float t = cy > 0 ? f[I] : -f[I]; my = fy ? t : my + t; fy = false; }
-                if constexpr (cz != 0) { float t = cz > 0 ? f[I] : -f[I]; mz = fz ? t : mz + t; fz = false; }
-            });
-            const float den = rho > 1e-4f ? rho : 1e-4f;
-            float ux = mx / den, uy = my / den, uz = mz / den;
-            // Stability guard (wgsl.ts:238-243).
-            const float ulen = __builtin_sqrtf(dot3(ux, uy, uz));
-            if (!__builtin_isfinite(rho) || !__builtin_isfinite(ux) || !__builtin_isfinite(uy) ||
-                !__builtin_isfinite(uz) || rho < 0.3f || rho > 3.0f || ulen > 0.35f) {
-                rho = 1.0f;
-                ux = a.u_in * 0.5f; uy = 0.0f; uz = 0.0f;
-                const float gu15 = 1.5f * dot3(ux, uy, uz);
+            float rho, ux, uy, uz;
+            if constexpr (EXACT) {
+                // Moments (wgsl.ts:231-236).
+                float mx = 0.f, my = 0.f, mz = 0.f;
+                rho = 0.f;
+                bool fx = true, fy = true, fz = true;
                 for19([&](auto ic) {
                     constexpr int I = decltype(ic)::value;
-                    f[I] = feq_c<I>(wt(I) * rho, ux, uy, uz, gu15);
+                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                    rho = (I == 0) ? f[0] : rho + f[I];
+                    if constexpr (cx != 0) { float t = cx > 0 ? f[I] : -f[I]; mx = fx ? t : mx + t; fx = false; }
+                    if constexpr (cy != 0) { float t = cy > 0 ? f[I] : -f[I]; my = fy ? t : my + t; fy = false; }
+                    if constexpr (cz != 0) { float t = cz > 0 ? f[I] : -f[I]; mz = fz ? t : mz + t; fz = false; }
                 });
-                g0x = g0y = g0z = g1x = g1y = g1z = g2x = g2y = g2z = 0.f;
-                guard = true;
+                const float den = rho > 1e-4f ? rho : 1e-4f;
+                ux = mx / den; uy = my / den; uz = mz / den;
+                // Stability guard (wgsl.ts:238-243).
+                const float ulen = __builtin_sqrtf(dot3(ux, uy, uz));
+                if (!__builtin_isfinite(rho) || !__builtin_isfinite(ux) || !__builtin_isfinite(uy) ||
+                    !__builtin_isfinite(uz) || rho < 0.3f || rho > 3.0f || ulen > 0.35f) {
+                    rho = 1.0f;
+                    ux = a.u_in * 0.5f; uy = 0.0f; uz = 0.0f;
+                    const float gu15 = 1.5f * dot3(ux, uy, uz);
+                    for19([&](auto ic) {
+                        constexpr int I = decltype(ic)::value;
+                        f[I] = feq_c<I>(wt(I) * rho, ux, uy, uz, gu15);
+                    });
+                    g0x = g0y = g0z = g1x = g1y = g1z = g2x = g2y = g2z = 0.f;
+                    guard = true;
+                }
+                // Duct relaxation (wgsl.ts:248): mix(u, (0.55 uIn, u.y, u.z), 0.04).
+                if (duct) {
+                    const float t = 0.04f, omt = 1.0f - t;
+                    const float bx = 0.55f * a.u_in;
+                    ux = ux * omt + bx * t;
+                    uy = uy * omt + uy * t;
+                    uz = uz * omt + uz * t;
+                }
+                // Non-equilibrium stress (wgsl.ts:250-257).
+                const float uu15 = 1.5f * dot3(ux, uy, uz);
+                float eq[19];
+                float pxx = 0.f, pyy = 0.f, pzz = 0.f, pxy = 0.f, pxz = 0.f, pyz = 0.f;
+                bool bxx = true, byy = true, bzz = true, bxy = true, bxz = true, byz = true;
+                for19([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                    eq[I] = feq_c<I>(wt(I) * rho, ux, uy, uz, uu15);
+                    const float ne = f[I] - eq[I];
+                    if constexpr (cx != 0) { pxx = bxx ? ne : pxx + ne; bxx = false; }
+                    if constexpr (cy != 0) { pyy = byy ? ne : pyy + ne; byy = false; }
+                    if constexpr (cz != 0) { pzz = bzz ? ne : pzz + ne; bzz = false; }
+                    if constexpr (cx * cy != 0) { float t = cx * cy > 0 ? ne : -ne; pxy = bxy ? t : pxy + t; bxy = false; }
+                    if constexpr (cx * cz != 0) { float t = cx * cz > 0 ? ne : -ne; pxz = bxz ? t : pxz + t; bxz = false; }
+                    if constexpr (cy * cz != 0) { float t = cy * cz > 0 ? ne : -ne; pyz = byz ? t : pyz + t; byz = false; }
+                });
+                // Smagorinsky relaxation time with the inlet/outlet sponge (wgsl.ts:258-262).
+                const float q = __builtin_sqrtf(((pxx * pxx + pyy * pyy) + pzz * pzz) +
+                                                2.0f * ((pxy * pxy + pxz * pxz) + pyz * pyz));
+                const float tau0 = *at(a.tau_x, (uint32_t)x);
+                const float tau = 0.5f * (tau0 + __builtin_sqrtf(tau0 * tau0 + a.smag_c * q / rho));
+                const float omega = 1.0f / tau;
+                const float om45 = (1.0f - omega) * 4.5f;
+                // Regularized collision (wgsl.ts:265-271).
+                const float third = (float)(1.0 / 3.0);
+                const float kA = 1.0f - third, kB = 0.0f - third;
+                for19([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                    float qp = ((cx != 0 ? kA : kB) * pxx + (cy != 0 ? kA : kB) * pyy) + (cz != 0 ? kA : kB) * pzz;
+                    if constexpr (cx * cy != 0) qp = qp + 2.0f * (cx * cy > 0 ? pxy : -pxy);
+                    if constexpr (cx * cz != 0) qp = qp + 2.0f * (cx * cz > 0 ? pxz : -pxz);
+                    if constexpr (cy * cz != 0) qp = qp + 2.0f * (cy * cz > 0 ? pyz : -pyz);
+                    const float v = eq[I] + (om45 * wt(I)) * qp;
+                    ST::bst(rout, idx * ESZ, (uint32_t)I * pbytes, v, wt(I));
+                });
+            } else {
+                // ---- moment-space regularized collision ----
+                // Pair sums/differences of opposite directions.
+                const float p1 = f[1] + f[2], d1 = f[1] - f[2];
+                const float p3 = f[3] + f[4], d3 = f[3] - f[4];
+                const float p5 = f[5] + f[6], d5 = f[5] - f[6];
+                const float p7 = f[7] + f[8], d7 = f[7] - f[8];
+                const float p9 = f[9] + f[10], d9 = f[9] - f[10];
+                const float p11 = f[11] + f[12], d11 = f[11] - f[12];
+                const float p13 = f[13] + f[14], d13 = f[13] - f[14];
+                const float p15 = f[15] + f[16], d15 = f[15] - f[16];
+                const float p17 = f[17] + f[18], d17 = f[17] - f[18];
+                // zeroth moment (sum of the working values: rho, or rho - 1 in deviation form)
+                const float r_s = ((f[0] + p1) + (p3 + p5)) + ((p7 + p9) + (p11 + p13)) + (p15 + p17);
+                float jx = (d1 + d7) + (d9 + d11) + d13;
+                float jy = (d3 + d7) - d9 + (d15 + d17);
+                float jz = (d5 + d11) - d13 + (d15 - d17);
+                // second moments of the working values
+                const float sxx = (p1 + p7) + (p9 + p11) + p13;
+                const float syy = (p3 + p7) + (p9 + p15) + p17;
+                const float szz = (p5 + p11) + (p13 + p15) + p17;
+                const float sxy = p7 - p9, sxz = p11 - p13, syz = p15 - p17;
+                rho = DEV ? r_s + 1.0f : r_s;
+                float inv = __builtin_amdgcn_rcpf(rho > 1e-4f ? rho : 1e-4f);  // 1 ulp
+                ux = jx * inv; uy = jy * inv; uz = jz * inv;
+                float rb = r_s;  // rho, or rho - 1 in deviation form
+                float pxx, pyy, pzz, pxy, pxz, pyz;
+                const float uu = __builtin_fmaf(ux, ux, __builtin_fmaf(uy, uy, uz * uz));
+                // Stability guard (wgsl.ts:238-243): NaN fails both comparisons.
+                if (!(rho >= 0.3f && rho <= 3.0f && uu <= 0.1225f)) {
+                    rho = 1.0f; rb = DEV ? 0.0f : 1.0f; inv = 1.0f;
+                    ux = a.u_in * 0.5f; uy = 0.0f; uz = 0.0f;
+                    jx = ux; jy = 0.0f; jz = 0.0f;
+                    pxx = pyy = pzz = pxy = pxz = pyz = 0.0f;  // f = feq exactly
+                    g0x = g0y = g0z = g1x = g1y = g1z = g2x = g2y = g2z = 0.f;
+                    guard = true;
+                } else {
+                    if (duct) {  // wgsl.ts:248, then the equilibrium uses the relaxed u
+                        const float t = 0.04f, omt = 1.0f - t;
+                        ux = __builtin_fmaf(ux, omt, 0.55f * a.u_in * t);
+                        jx = rho * ux;
+                        jy = rho * uy;
+                        jz = rho * uz;
+                    }
+                    // Pi^neq = S - rho (u u + I/3)  (= sum_i (f_i - feq_i) c c)
+                    const float r3 = rb * (float)(1.0 / 3.0);
+                    pxx = __builtin_fmaf(-jx, ux, sxx - r3);
+                    pyy = __builtin_fmaf(-jy, uy, syy - r3);
+                    pzz = __builtin_fmaf(-jz, uz, szz - r3);
+                    pxy = __builtin_fmaf(-jx, uy, sxy);
+                    pxz = __builtin_fmaf(-jx, uz, sxz);
+                    pyz = __builtin_fmaf(-jy, uz, syz);
+                }
+                // Smagorinsky relaxation time with the inlet/outlet sponge (wgsl.ts:258-262).
+                const float q2 = __builtin_fmaf(pxx, pxx, __builtin_fmaf(pyy, pyy, pzz * pzz)) +
+                                 2.0f * __builtin_fmaf(pxy, pxy, __builtin_fmaf(pxz, pxz, pyz * pyz));
+                const float q = __builtin_amdgcn_sqrtf(q2);
+                const float tau0 = *at(a.tau_x, (uint32_t)x);
+                const float tau = 0.5f * (tau0 + __builtin_amdgcn_sqrtf(__builtin_fmaf(tau0, tau0, a.smag_c * q * inv)));
+                const float k1 = 1.0f - __builtin_amdgcn_rcpf(tau);  // 1 - omega
+                // M = j u + (1 - omega) Pi;  A = rho - 1.5 (j.u + (1 - omega) tr Pi)
+                const float mxx = __builtin_fmaf(k1, pxx, jx * ux);
+                const float myy = __builtin_fmaf(k1, pyy, jy * uy);
+                const float mzz = __builtin_fmaf(k1, pzz, jz * uz);
+                const float mxy = __builtin_fmaf(k1, pxy, jx * uy);
+                const float mxz = __builtin_fmaf(k1, pxz, jx * uz);
+                const float myz = __builtin_fmaf(k1, pyz, jy * uz);
+                const float Aw = __builtin_fmaf(-1.5f, (mxx + myy) + mzz, rb);
+                const float tx3 = 3.0f * jx, ty3 = 3.0f * jy, tz3 = 3.0f * jz;
+                // fout_i - (w_i in deviation form) = w_i (A + 3 c.j + 4.5 cc:M)
+                float o[19];
+                o[0] = kW0 * Aw;
+                const float kx = __builtin_fmaf(4.5f, mxx, Aw), ky = __builtin_fmaf(4.5f, myy, Aw),
+                            kz = __builtin_fmaf(4.5f, mzz, Aw);
+                o[1] = kW1 * (kx + tx3); o[2] = kW1 * (kx - tx3);
+                o[3] = kW1 * (ky + ty3); o[4] = kW1 * (ky - ty3);
+                o[5] = kW1 * (kz + tz3); o[6] = kW1 * (kz - tz3);
+                const float dxy = __builtin_fmaf(4.5f, mxx + myy, Aw), exy = 9.0f * mxy;
+                const float dxz = __builtin_fmaf(4.5f, mxx + mzz, Aw), exz = 9.0f * mxz;
+                const float dyz = __builtin_fmaf(4.5f, myy + mzz, Aw), eyz = 9.0f * myz;
+                const float axyp = dxy + exy, axym = dxy - exy, sxyp = tx3 + ty3, sxym = tx3 - ty3;
+                o[7] = kW2 * (axyp + sxyp); o[8] = kW2 * (axyp - sxyp);
+                o[9] = kW2 * (axym + sxym); o[10] = kW2 * (axym - sxym);
+                const float axzp = dxz + exz, axzm = dxz - exz, sxzp = tx3 + tz3, sxzm = tx3 - tz3;
+                o[11] = kW2 * (axzp + sxzp); o[12] = kW2 * (axzp - sxzp);
+                o[13] = kW2 * (axzm + sxzm); o[14] = kW2 * (axzm - sxzm);
+                const float ayzp = dyz + eyz, ayzm = dyz - eyz, syzp = ty3 + tz3, syzm = ty3 - tz3;
+                o[15] = kW2 * (ayzp + syzp); o[16] = kW2 * (ayzp - syzp);
+                o[17] = kW2 * (ayzm + syzm); o[18] = kW2 * (ayzm - syzm);
+                for19([&](auto ic) {
+                    constexpr int I = decltype(ic)::value;
+                    SV(idx * ESZ, (uint32_t)I * pbytes, o[I], wt(I));
+                });
             }
-            // Duct relaxation (wgsl.ts:248): mix(u, (0.55 uIn, u.y, u.z), 0.04).
-            if (duct) {
-                const float t = 0.04f, omt = 1.0f - t;
-                const float bx = 0.55f * a.u_in;
-                ux = ux * omt + bx * t;
-                uy = uy * omt + uy * t;
-                uz = uz * omt + uz * t;
-            }
-            // Non-equilibrium stress (wgsl.ts:250-257).
-            const float uu15 = 1.5f * dot3(ux, uy, uz);
-            float eq[19];
-            float pxx = 0.f, pyy = 0.f, pzz = 0.f, pxy = 0.f, pxz = 0.f, pyz = 0.f;
-            bool bxx = true, byy = true, bzz = true, bxy = true, bxz = true, byz = true;
-            for19([&](auto ic) {
-                constexpr int I = decltype(ic)::value;
-                constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
-                eq[I] = feq_c<I>(wt(I) * rho, ux, uy, uz, uu15);
-                const float ne = f[I] - eq[I];
-                if constexpr (cx != 0) { pxx = bxx ? ne : pxx + ne; bxx = false; }
-                if constexpr (cy != 0) { pyy = byy ? ne : pyy + ne; byy = false; }
-                if constexpr (cz != 0) { pzz = bzz ? ne : pzz + ne; bzz = false; }
-                if constexpr (cx * cy != 0) { float t = cx * cy > 0 ? ne : -ne; pxy = bxy ? t : pxy + t; bxy = false; }
-                if constexpr (cx * cz != 0) { float t = cx * cz > 0 ? ne : -ne; pxz = bxz ? t : pxz + t; bxz = false; }
-                if constexpr (cy * cz != 0) { float t = cy * cz > 0 ? ne : -ne; pyz = byz ? t : pyz + t; byz = false; }
-            });
-            // Smagorinsky relaxation time with the inlet/outlet sponge (wgsl.ts:258-262).
-            const float q = __builtin_sqrtf(((pxx * pxx + pyy * pyy) + pzz * pzz) +
-                                            2.0f * ((pxy * pxy + pxz * pxz) + pyz * pyz));
-            const float tau0 = *at(a.tau_x, (uint32_t)x);
-            const float tau = 0.5f * (tau0 + __builtin_sqrtf(tau0 * tau0 + a.smag_c * q / rho));
-            const float omega = 1.0f / tau;
-            const float om45 = (1.0f - omega) * 4.5f;
-            // Regularized collision (wgsl.ts:265-271).
-            const float third = (float)(1.0 / 3.0);
-            const float kA = 1.0f - third, kB = 0.0f - third;
-            for19([&](auto ic) {
-                constexpr int I = decltype(ic)::value;
-                constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
-                float qp = ((cx != 0 ? kA : kB) * pxx + (cy != 0 ? kA : kB) * pyy) + (cz != 0 ? kA : kB) * pzz;
-                if constexpr (cx * cy != 0) qp = qp + 2.0f * (cx * cy > 0 ? pxy : -pxy);
-                if constexpr (cx * cz != 0) qp = qp + 2.0f * (cx * cz > 0 ? pxz : -pxz);
-                if constexpr (cy * cz != 0) qp = qp + 2.0f * (cy * cz > 0 ? pyz : -pyz);
-                const float v = eq[I] + (om45 * wt(I)) * qp;
-                ST::bst(rout, idx * ESZ, (uint32_t)I * pbytes, v, wt(I));
-            });
             if (a.write_fields) a.fields[idx] = make_float4(ux, uy, uz, rho);
             if (last_tile && outlet) { a.prev_u[pidx] = ux; a.prev_u[pidx + 1] = uy; a.prev_u[pidx + 2] = uz; }
         }
@@ -389,7 +589,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
             gc += __shfl_xor(gc, o, 64);
         }
         if (lane == 0) {
-            TileSlot* s = a.slots + ((size_t)l * kTileY + wv);
+            TileSlot* s = a.slots + ((size_t)l * kWaves + wv);
 #pragma unroll
             for (int k = 0; k < 9; k++) s->d[k] += d[k];
             const int qmap[6] = {0, 1, 3, 4, 6, 7};
@@ -538,6 +738,7 @@ void fill_step_consts(const aero_ctx* c, StepArgs* a) {
     a->smag_c = (float)(18.0 * 1.41421356) * c->sim.smag * c->sim.smag;
     for (int i = 0; i < 19; i++) {
         a->feq_in[i] = feq_host(i, 1.0f, c->sim.u_in, 0.0f, 0.0f);
+        a->feq_in_dev[i] = a->feq_in[i] - wt(i);
         a->floor_term[i] = 6.0f * wt(i) * (float)kCX[i] * c->sim.u_in;
     }
     a->flags = c->flags;
@@ -574,10 +775,20 @@ hipError_t launch_step(aero_ctx* c, int write_fields) {
     a.fout = c->f[c->cur ^ 1];
     a.write_fields = write_fields;
     const dim3 grid(c->ntiles), block(kBlock);
+    const bool exact = c->exact != 0;
     switch (c->storage) {
-    case AERO_STORE_F32: hipLaunchKernelGGL(k_step<AERO_STORE_F32>, grid, block, 0, c->stream, a); break;
-    case AERO_STORE_F16S: hipLaunchKernelGGL(k_step<AERO_STORE_F16S>, grid, block, 0, c->stream, a); break;
-    default: hipLaunchKernelGGL(k_step<AERO_STORE_BF16S>, grid, block, 0, c->stream, a); break;
+    case AERO_STORE_F32:
+        if (exact) hipLaunchKernelGGL((k_step<AERO_STORE_F32, true>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_step<AERO_STORE_F32, false>), grid, block, 0, c->stream, a);
+        break;
+    case AERO_STORE_F16S:
+        if (exact) hipLaunchKernelGGL((k_step<AERO_STORE_F16S, true>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_step<AERO_STORE_F16S, false>), grid, block, 0, c->stream, a);
+        break;
+    default:
+        if (exact) hipLaunchKernelGGL((k_step<AERO_STORE_BF16S, true>), grid, block, 0, c->stream, a);
+        else hipLaunchKernelGGL((k_step<AERO_STORE_BF16S, false>), grid, block, 0, c->stream, a);
+        break;
     }
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) c->cur ^= 1;
@@ -585,7 +796,7 @@ hipError_t launch_step(aero_ctx* c, int write_fields) {
 }
 
 hipError_t launch_reduce_slots(aero_ctx* c) {
-    hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->slots, c->ntiles * (uint32_t)kTileY, (TileSlot*)c->red_out);
+    hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->slots, c->ntiles * (uint32_t)kWaves, (TileSlot*)c->red_out);
     return hipGetLastError();
 }
 

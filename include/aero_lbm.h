@@ -38,6 +38,12 @@ extern "C" {
 #define AERO_STORE_F32 0   /* plain f32, 153 B per cell update */
 #define AERO_STORE_F16S 1  /* f16 of (f_i - w_i), 77 B per cell update */
 #define AERO_STORE_BF16S 2 /* bf16 of (f_i - w_i), 77 B per cell update */
+/* OR-ed into the storage argument of aero_create: evaluate the collision with the
+ * reference's per-direction operation order and no FMA contraction
+ * (src/engine/wgsl.ts:231-271), bit-identical to an f32 evaluation of the WGSL.
+ * Without it the library uses the algebraically identical moment-space form with
+ * FMAs (about a third of the arithmetic), equal to the reference within f32 rounding. */
+#define AERO_MATH_EXACT 0x100
 
 /* == WGSL struct Grid (wgsl.ts:6), 32 bytes.  n must equal nx*ny*nz. */
 typedef struct { uint32_t nx, ny, nz, n; float ox, oy, oz, h; } aero_grid;
@@ -98,7 +104,8 @@ const char* aero_global_error(void);
 const char* aero_last_error(const aero_ctx* ctx);
 
 /* Allocates the tunnel on the current HIP device (engine.ts:226-246).  Populations
- * are left uninitialised: call aero_init.  storage is AERO_STORE_*. */
+ * are left uninitialised: call aero_init.  storage is AERO_STORE_*, optionally
+ * OR-ed with AERO_MATH_EXACT. */
 int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ctx** out);
 int aero_destroy(aero_ctx* ctx);
 

@@ -21,8 +21,9 @@ SDF = orc.SDFOracle()
 REL = 1e-5
 
 
-def engine_for(dims, h=0.1, origin=(0.0, 0.0, 0.0), u=0.05, tau0=0.51, smag=0.1, storage="f32"):
-    return aerolbm.HipAeroEngine(dims=dims, h=h, origin=origin, u_lb=u, tau0=tau0, smag=smag, storage=storage)
+def engine_for(dims, h=0.1, origin=(0.0, 0.0, 0.0), u=0.05, tau0=0.51, smag=0.1, storage="f32", math="exact"):
+    return aerolbm.HipAeroEngine(dims=dims, h=h, origin=origin, u_lb=u, tau0=tau0, smag=smag, storage=storage,
+                                 math=math)
 
 
 def ogrid(e):
@@ -61,11 +62,12 @@ def perturbed_init(e, rng, amp=0.02):
     return f, fields0
 
 
+@pytest.mark.parametrize("math", ["exact", "fast"])
 @pytest.mark.parametrize("steps", [1, 100])
-def test_step_matches_f32_oracle_mixed_geometry(steps):
+def test_step_matches_f32_oracle_mixed_geometry(steps, math):
     rng = np.random.default_rng(10)
     dims = (48, 20, 12)
-    e = engine_for(dims, u=0.06)
+    e = engine_for(dims, u=0.06, math=math)
     solid, emit = mixed_geometry(dims, rng)
     e.set_flags(solid, emit)
     e.reset()
@@ -74,17 +76,21 @@ def test_step_matches_f32_oracle_mixed_geometry(steps):
     e.step(steps)
     got = e.populations()
     want, wfields, st = F32.run(ogrid(e), osim(e), f, solid, fields0, emit, steps)
-    print(f"steps={steps} max rel err {rel_err(got, want):.3e}, exact={np.array_equal(got, want)}")
+    print(f"{math} steps={steps} max rel err {rel_err(got, want):.3e}, exact={np.array_equal(got, want)}")
     assert rel_err(got, want) <= REL
-    assert np.array_equal(got, want)  # the f32 arithmetic is the reference's, term for term
+    if math == "exact":
+        assert np.array_equal(got, want)  # the f32 arithmetic is the reference's, term for term
     gf = e.fields()
     assert rel_err(gf[:, 3], wfields[:, 3]) <= REL
-    assert np.abs(gf[:, :3] - wfields[:, :3]).max() <= 1e-7
+    assert np.abs(gf[:, :3] - wfields[:, :3]).max() <= (1e-7 if math == "exact" else 2e-6)
     d, q, nsteps = e.read_forces_raw(reset=False)
     assert nsteps == steps
-    assert np.array_equal(q, np.array(st.forces_q)[:9])
     od = np.array(st.forces_d)
-    assert np.abs(d - od).max() <= 1e-9 * max(1.0, np.abs(od).max())
+    if math == "exact":
+        assert np.array_equal(q, np.array(st.forces_q)[:9])
+        assert np.abs(d - od).max() <= 1e-9 * max(1.0, np.abs(od).max())
+    else:  # forces sum f32 link values; 1e-5 of the largest component
+        assert np.abs(d - od).max() <= 1e-5 * np.abs(od).max()
     assert np.abs(od[[0, 3, 6]]).min() > 0  # all three groups carry force
     assert e.guard_count() == st.guard_hits
 
@@ -98,12 +104,13 @@ def test_init_matches_oracle():
     assert not e.temps().any()
 
 
+@pytest.mark.parametrize("math", ["exact", "fast"])
 @pytest.mark.parametrize("steps", [1, 100])
-def test_cube_workload_full_grid(steps):
+def test_cube_workload_full_grid(steps, math):
     """The 128x64x64 cube workload: voxelize on the GPU, step, compare with the
     oracle voxelizer + f32 oracle (OpenMP build; per-cell arithmetic is identical)."""
     wl = W.cube()
-    e = aerolbm.HipAeroEngine(**wl.engine_kwargs())
+    e = aerolbm.HipAeroEngine(math=math, **wl.engine_kwargs())
     e.set_shape(wl.shape)
     e.reset()
     solid, emit = e.flags()
@@ -114,11 +121,15 @@ def test_cube_workload_full_grid(steps):
     f0, fields0 = F32P.init(ogrid(e), osim(e))
     want, wfields, st = F32P.run(ogrid(e), osim(e), f0, osolid, fields0, oemit, steps)
     got = e.populations()
-    print(f"cube steps={steps} max rel err {rel_err(got, want):.3e}")
+    print(f"cube {math} steps={steps} max rel err {rel_err(got, want):.3e}")
     assert rel_err(got, want) <= REL
     d, q, _ = e.read_forces_raw()
-    assert np.array_equal(q, np.array(st.forces_q)[:9])
-    assert abs(d[0] - st.forces_d[0]) <= 1e-9 * abs(st.forces_d[0])
+    if math == "exact":
+        assert np.array_equal(got, want)
+        assert np.array_equal(q, np.array(st.forces_q)[:9])
+        assert abs(d[0] - st.forces_d[0]) <= 1e-9 * abs(st.forces_d[0])
+    else:
+        assert abs(d[0] - st.forces_d[0]) <= 1e-5 * abs(st.forces_d[0])
 
 
 @pytest.mark.parametrize("kind", ["cube", "sphere", "wing", "empty", "custom", "scene"])
@@ -160,10 +171,11 @@ def test_voxelize_bit_exact(kind):
         assert not solid.any()
 
 
-def test_heat_matches_oracle_after_steps():
+@pytest.mark.parametrize("math", ["exact", "fast"])
+def test_heat_matches_oracle_after_steps(math):
     rng = np.random.default_rng(11)
     dims = (48, 20, 12)
-    e = engine_for(dims, u=0.08)
+    e = engine_for(dims, u=0.08, math=math)
     solid, emit = mixed_geometry(dims, rng)
     e.set_flags(solid, emit)
     e.reset()
@@ -180,12 +192,14 @@ def test_heat_matches_oracle_after_steps():
     got = e.temps()
     print(f"heat max abs err {np.abs(got - t2).max():.3e}")
     assert np.abs(got - t2).max() <= REL * max(1.0, np.abs(t2).max())
-    assert np.array_equal(got, t2)
+    if math == "exact":
+        assert np.array_equal(got, t2)
 
 
-def test_guard_fires_on_injected_nan():
+@pytest.mark.parametrize("math,storage", [("exact", "f32"), ("fast", "f32"), ("fast", "f16s")])
+def test_guard_fires_on_injected_nan(math, storage):
     dims = (32, 12, 10)
-    e = engine_for(dims, u=0.06)
+    e = engine_for(dims, u=0.06, math=math, storage=storage)
     e.reset()
     f = e.populations()
     x, y, z = 16, 6, 5
@@ -195,18 +209,20 @@ def test_guard_fires_on_injected_nan():
     e.step(1)
     got = e.populations()
     want = np.array([F32.feq(i, 1.0, np.float32(0.06) * np.float32(0.5), 0.0, 0.0) for i in range(19)], np.float32)
-    assert np.array_equal(got[:, idx], want)
+    tol = {"f32": 0.0 if math == "exact" else 1e-7, "f16s": 2e-5}[storage]
+    assert np.abs(got[:, idx] - want).max() <= tol
     assert e.guard_count() == 1
     assert np.isfinite(got).all()
 
 
-@pytest.mark.parametrize("storage,bound", [("f16s", 2e-3), ("bf16s", 1.5e-2)])
-def test_shifted_storage_error_vs_fp64(storage, bound):
+@pytest.mark.parametrize("math", ["exact", "fast"])
+@pytest.mark.parametrize("storage,bound", [("f32", 1e-5), ("f16s", 2e-3), ("bf16s", 1.5e-2)])
+def test_storage_error_vs_fp64(storage, bound, math):
     """f16/bf16 of (f - w) with f32 collision, 200 steps of the cube workload at a
     reduced grid: population and force error against the FP64 oracle."""
     dims = (64, 32, 24)
     h = 0.1
-    e = engine_for(dims, h=h, origin=(0.6, 0.0, 0.0), u=0.05, storage=storage)
+    e = engine_for(dims, h=h, origin=(0.6, 0.0, 0.0), u=0.05, storage=storage, math=math)
     s = W.base_shape(aerolbm.ShapeKind.Cube)
     e.set_shape(s)
     e.reset()
@@ -220,7 +236,7 @@ def test_shifted_storage_error_vs_fp64(storage, bound):
     perr = np.abs(got - want).max() / np.abs(want).max()
     d, _, _ = e.read_forces_raw()
     ferr = abs(d[0] - st.forces_d[0]) / abs(st.forces_d[0])
-    print(f"{storage}: population err {perr:.2e} (of max), drag-force err {ferr:.2e}")
+    print(f"{storage}/{math}: population err {perr:.2e} (of max), drag-force err {ferr:.2e}")
     assert perr < bound
     assert ferr < 0.01
 
