@@ -91,6 +91,9 @@ def main() -> None:
     e = aerolbm.HipAeroEngine(storage=args.storage, math=args.math, **wl.engine_kwargs())
     e.set_shape(wl.shape, prims=wl.prims)
     e.reset()
+    shape, tuned_us = e.autotune(steps_per_shape=10)  # launch-shape autotuner (state is restored)
+    launch = e.launch_shape()
+    print(f"[bench] autotuned launch shape {launch} ({tuned_us:.1f} us/step)", file=sys.stderr, flush=True)
 
     frame_steps = 8
     heat_dt = frame_steps / 2
@@ -141,6 +144,7 @@ def main() -> None:
             "cells": wl.cells,
             "storage": args.storage,
             "math": args.math,
+            "launch": launch,
         },
         "roofline": {
             "bound": "hbm",

@@ -89,6 +89,10 @@ _PROTOS = {
     "aero_guard_count": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "aero_sync": (C.c_int, [_P]),
     "aero_profile_begin": (C.c_int, [_P, _U32]),
+    "aero_autotune": (C.c_int, [_P, _U32, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+    "aero_set_launch_shape": (C.c_int, [_P, C.c_int]),
+    "aero_get_launch_shape": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_int)]),
     "aero_profile_end": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
 }
 

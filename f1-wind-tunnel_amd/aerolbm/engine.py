@@ -230,6 +230,23 @@ class HipAeroEngine:
         self.steps_since_read += int(nsteps)
         return float(ms.value)
 
+    def autotune(self, steps_per_shape: int = 20):
+        """Times every stream-collide launch shape on this tunnel (state is restored)
+        and keeps the fastest.  Returns (shape index, us per step; -1 if cached)."""
+        sh = C.c_int()
+        us = C.c_float()
+        self._c(self._lib.aero_autotune(self.ctx, int(steps_per_shape), C.byref(sh), C.byref(us)), "aero_autotune")
+        return int(sh.value), float(us.value)
+
+    def set_launch_shape(self, shape: int) -> None:
+        self._c(self._lib.aero_set_launch_shape(self.ctx, int(shape)), "aero_set_launch_shape")
+
+    def launch_shape(self) -> dict:
+        sh, tx, swz, per = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        self._c(self._lib.aero_get_launch_shape(self.ctx, C.byref(sh), C.byref(tx), C.byref(swz), C.byref(per)),
+                "aero_get_launch_shape")
+        return {"shape": sh.value, "tile_x": tx.value, "xcd_order": bool(swz.value), "persistent": bool(per.value)}
+
     def profile_begin(self, max_calls: int = 4096) -> None:
         self._c(self._lib.aero_profile_begin(self.ctx, int(max_calls)), "aero_profile_begin")
 

@@ -37,6 +37,7 @@ constexpr int kTileX = 64;
 constexpr int kTileY = 4;
 constexpr int kBlock = kTileX * kTileY;
 constexpr int kWaves = kBlock / 64;  // force slots per tile (one per wavefront)
+constexpr uint32_t kPersistBlocksPerCU = 6;  // resident 256-thread blocks per CU at the kernel's VGPR use
 static_assert(kBlock == 256 && kTileX % 64 == 0, "tile must be 256 threads, whole wavefronts per row");
 
 // Force/guard partial sums owned by one tile.  Each tile adds its own contribution
@@ -48,6 +49,12 @@ struct alignas(16) TileSlot {
     unsigned long long guard;
     unsigned long long pad;
 };
+
+// Launch shape of the stream-collide kernel (see k_step): tile width, XCD-ordered
+// tiles, persistent grid.
+struct LaunchShape { int tx; bool swz; bool persist; };
+constexpr int kNumShapes = 10;
+extern const LaunchShape kShapes[kNumShapes];
 
 struct StepArgs {
     const void* fin;
@@ -93,8 +100,10 @@ struct aero_ctx {
     int tcur;
     float* tau_x;
     aero::TileSlot* slots;
-    uint32_t tiles_x, tiles_y, ntiles;
-    void* red_out;          // device reduction result (TileSlot)
+    uint32_t nslots;        // slots for the largest tiling of any launch shape
+    int shape;              // index into kShapes used by aero_step
+    uint32_t persist_blocks;  // grid of the persistent launch shapes (fills the chip once)
+    void* red_out;          // device reduction scratch: kRedParts partials + the result
     uint64_t steps;         // steps since the last force reset
     // W17 car model (kind 5)
     uint16_t* car_body;
@@ -116,6 +125,9 @@ namespace aero {
 hipError_t launch_init(aero_ctx* c);
 hipError_t launch_flags(aero_ctx* c);
 hipError_t launch_step(aero_ctx* c, int write_fields);
+hipError_t launch_step_shape(aero_ctx* c, int write_fields, int shape);
+void tile_geometry(const aero_ctx* c, int shape, uint32_t* tiles_x, uint32_t* tiles_y, uint32_t* ntiles);
+constexpr uint32_t kRedParts = 256;
 hipError_t launch_reduce_slots(aero_ctx* c);
 hipError_t launch_heat(aero_ctx* c, const aero_heat* h);
 hipError_t launch_voxelize(aero_ctx* c, const aero_shape* s, const float* d_profile,

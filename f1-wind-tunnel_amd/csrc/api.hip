@@ -128,9 +128,15 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
         delete c;
         return fail(nullptr, AERO_E_INVALID, "aero_create: a population array must stay below 4 GiB");
     }
-    c->tiles_x = (grid->nx + kTileX - 1) / kTileX;
-    c->tiles_y = (grid->ny + kTileY - 1) / kTileY;
-    c->ntiles = c->tiles_x * c->tiles_y * grid->nz;
+    c->nslots = 0;
+    for (int sh = 0; sh < kNumShapes; sh++) {
+        uint32_t tx, ty, nt;
+        tile_geometry(c, sh, &tx, &ty, &nt);
+        if (nt * (uint32_t)kWaves > c->nslots) c->nslots = nt * (uint32_t)kWaves;
+    }
+    // Untuned default: narrow tiles for small tunnels, full 256-cell rows for large ones
+    // (aero_autotune replaces it with a measured choice).
+    c->shape = (n > 8000000ull) ? 6 : 0;
     const size_t outlet = (size_t)grid->ny * grid->nz * 3;
 
     struct Alloc { void** p; size_t bytes; } allocs[] = {
@@ -144,8 +150,8 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
         {(void**)&c->temps[0], n * 4},
         {(void**)&c->temps[1], n * 4},
         {(void**)&c->tau_x, (size_t)grid->nx * 4},
-        {(void**)&c->slots, (size_t)c->ntiles * kWaves * sizeof(TileSlot)},
-        {&c->red_out, sizeof(TileSlot)},
+        {(void**)&c->slots, (size_t)c->nslots * sizeof(TileSlot)},
+        {&c->red_out, (kRedParts + 1) * sizeof(TileSlot)},
     };
     for (auto& a : allocs) {
         e = hipMalloc(a.p, a.bytes);
@@ -162,11 +168,18 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
         set_global_error(std::string("aero_create: stream/event: ") + hipGetErrorString(e));
         return AERO_E_HIP;
     }
+    {
+        // Persistent launch shapes fill the chip exactly once: CUs x resident blocks.
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess || cus <= 0)
+            cus = 256;
+        c->persist_blocks = (uint32_t)cus * kPersistBlocksPerCU;
+    }
     // Empty tunnel until the first voxelize / set_flags.
     e = hipMemsetAsync(c->solid, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->emit, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->flags, 0, c->ns, c->stream);
-    if (e == hipSuccess) e = hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * kWaves * sizeof(TileSlot), c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->slots, 0, (size_t)c->nslots * sizeof(TileSlot), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->temps[0], 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->temps[1], 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -205,7 +218,7 @@ int aero_init(aero_ctx* c) {
     const uint64_t n = c->grid.n;
     HIPCHK(c, hipMemsetAsync(c->temps[0], 0, n * 4, c->stream));
     HIPCHK(c, hipMemsetAsync(c->temps[1], 0, n * 4, c->stream));
-    HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * kWaves * sizeof(TileSlot), c->stream));
+    HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->nslots * sizeof(TileSlot), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->cur = 0;
     c->tcur = 0;
@@ -346,6 +359,98 @@ int aero_step_timed(aero_ctx* c, uint32_t nsteps, float* ms) {
     return AERO_OK;
 }
 
+namespace {
+struct TuneKey {
+    std::string dev;
+    uint32_t nx, ny, nz;
+    int storage, exact;
+    bool operator==(const TuneKey& o) const {
+        return dev == o.dev && nx == o.nx && ny == o.ny && nz == o.nz && storage == o.storage && exact == o.exact;
+    }
+};
+std::mutex g_tune_mu;
+std::vector<std::pair<TuneKey, int>> g_tune_cache;
+}  // namespace
+
+int aero_autotune(aero_ctx* c, uint32_t steps_per_shape, int* shape_out, float* us_per_step) {
+    CTXCHK(c);
+    if (steps_per_shape == 0 || steps_per_shape > 10000)
+        return fail(c, AERO_E_INVALID, "aero_autotune: steps_per_shape must be 1..10000");
+    hipDeviceProp_t prop;
+    HIPCHK(c, hipGetDeviceProperties(&prop, c->device));
+    const TuneKey key{prop.gcnArchName, c->grid.nx, c->grid.ny, c->grid.nz, c->storage, c->exact};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (auto& kv : g_tune_cache)
+            if (kv.first == key) {
+                c->shape = kv.second;
+                if (shape_out) *shape_out = c->shape;
+                if (us_per_step) *us_per_step = -1.0f;
+                return AERO_OK;
+            }
+    }
+    // The candidates run real steps on a copy of the state that is restored afterwards:
+    // populations, the outlet velocity plane and the force slots.
+    const size_t slot_bytes = (size_t)c->nslots * sizeof(TileSlot);
+    const size_t pu_bytes = (size_t)c->grid.ny * c->grid.nz * 3 * sizeof(float);
+    void *save_f = nullptr, *save_s = nullptr, *save_p = nullptr;
+    hipError_t e = hipMalloc(&save_f, c->pop_bytes);
+    if (e == hipSuccess) e = hipMalloc(&save_s, slot_bytes);
+    if (e == hipSuccess) e = hipMalloc(&save_p, pu_bytes);
+    const int cur0 = c->cur;
+    if (e == hipSuccess) e = hipMemcpyAsync(save_f, c->f[cur0], c->pop_bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(save_s, c->slots, slot_bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(save_p, c->prev_u, pu_bytes, hipMemcpyDeviceToDevice, c->stream);
+    int best = c->shape;
+    float best_us = 0.0f;
+    for (int sh = 0; sh < kNumShapes && e == hipSuccess; sh++) {
+        for (int k = 0; k < 2 && e == hipSuccess; k++) e = launch_step_shape(c, 0, sh);  // warm-up
+        if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+        for (uint32_t k = 0; k < steps_per_shape && e == hipSuccess; k++) e = launch_step_shape(c, 0, sh);
+        if (e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(c->ev1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        const float us = ms * 1000.0f / (float)steps_per_shape;
+        if (e == hipSuccess && (sh == 0 || us < best_us)) { best = sh; best_us = us; }
+    }
+    // restore the state (the populations go back to the buffer they came from)
+    if (e == hipSuccess) e = hipMemcpyAsync(c->f[cur0], save_f, c->pop_bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->slots, save_s, slot_bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->prev_u, save_p, pu_bytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    c->cur = cur0;
+    if (save_f) (void)hipFree(save_f);
+    if (save_s) (void)hipFree(save_s);
+    if (save_p) (void)hipFree(save_p);
+    if (e != hipSuccess) return hip_fail(c, e, "aero_autotune");
+    c->shape = best;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache.emplace_back(key, best);
+    }
+    if (shape_out) *shape_out = best;
+    if (us_per_step) *us_per_step = best_us;
+    return AERO_OK;
+}
+
+int aero_set_launch_shape(aero_ctx* c, int shape) {
+    CTXCHK(c);
+    if (shape < 0 || shape >= kNumShapes) return fail(c, AERO_E_INVALID, "aero_set_launch_shape: shape out of range");
+    c->shape = shape;
+    return AERO_OK;
+}
+
+int aero_get_launch_shape(aero_ctx* c, int* shape, int* tile_x, int* xcd_order, int* persistent) {
+    CTXCHK(c);
+    if (!shape) return fail(c, AERO_E_INVALID, "aero_get_launch_shape: null shape");
+    *shape = c->shape;
+    if (tile_x) *tile_x = kShapes[c->shape].tx;
+    if (xcd_order) *xcd_order = kShapes[c->shape].swz ? 1 : 0;
+    if (persistent) *persistent = kShapes[c->shape].persist ? 1 : 0;
+    return AERO_OK;
+}
+
 int aero_heat_step(aero_ctx* c, const aero_heat* h, uint32_t nsteps) {
     CTXCHK(c);
     if (!h) return fail(c, AERO_E_INVALID, "aero_heat_step: null heat");
@@ -357,8 +462,8 @@ int aero_read_forces(aero_ctx* c, double sum_fp64[9], int64_t sum_q[9], uint64_t
     CTXCHK(c);
     HIPCHK(c, launch_reduce_slots(c));
     TileSlot r;
-    HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(TileSlot), hipMemcpyDeviceToHost, c->stream));
-    if (reset) HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->ntiles * kWaves * sizeof(TileSlot), c->stream));
+    HIPCHK(c, hipMemcpyAsync(&r, (TileSlot*)c->red_out + kRedParts, sizeof(TileSlot), hipMemcpyDeviceToHost, c->stream));
+    if (reset) HIPCHK(c, hipMemsetAsync(c->slots, 0, (size_t)c->nslots * sizeof(TileSlot), c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (sum_fp64) for (int k = 0; k < 9; k++) sum_fp64[k] = r.d[k];
     if (sum_q) for (int k = 0; k < 9; k++) sum_q[k] = (int64_t)r.q[k];
@@ -372,7 +477,7 @@ int aero_guard_count(aero_ctx* c, uint64_t* count) {
     if (!count) return fail(c, AERO_E_INVALID, "aero_guard_count: null count");
     HIPCHK(c, launch_reduce_slots(c));
     TileSlot r;
-    HIPCHK(c, hipMemcpyAsync(&r, c->red_out, sizeof(TileSlot), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(&r, (TileSlot*)c->red_out + kRedParts, sizeof(TileSlot), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *count = r.guard;
     return AERO_OK;

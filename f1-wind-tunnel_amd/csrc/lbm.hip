@@ -178,8 +178,9 @@ __device__ __forceinline__ float feq_dev(float ux, float uy, float uz, float uu1
 //                  scalars.  Shifted storage stays in deviation form (f - w)
 //                  throughout, which also keeps the small departures from
 //                  equilibrium out of the cancellation against w.
-template <int S, bool EXACT>
-__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+template <int S, bool EXACT, int TX>
+__device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l) {
+    constexpr int TY = kBlock / TX;
     using ST = Store<S>;
     using T = typename ST::T;
     constexpr bool DEV = !EXACT && (S != AERO_STORE_F32);  // pulled values are f - w
@@ -188,7 +189,6 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     const rsrc_t rout = make_rsrc(a.fout, a.pop_bytes);
     const uint32_t pbytes = a.plane_bytes;  // bytes per population plane
 
-    const uint32_t l = xcd_swizzle(blockIdx.x, gridDim.x);
     const uint32_t txy = a.tiles_x * a.tiles_y;
     const uint32_t tz = l / txy;
     const uint32_t rem = l - tz * txy;
@@ -196,8 +196,8 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     const uint32_t tx = rem - ty * a.tiles_x;
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const int x = (int)tx * kTileX + (int)(threadIdx.x % kTileX);
-    const int y = (int)ty * kTileY + (int)(threadIdx.x / kTileX);  // wave-uniform
+    const int x = (int)tx * TX + (int)(threadIdx.x % TX);
+    const int y = (int)ty * TY + (int)(threadIdx.x / TX);  // wave-uniform (TX >= 64)
     const int z = (int)tz;                // block-uniform
     const int NX = (int)a.nx, NY = (int)a.ny, NZ = (int)a.nz;
     const bool active = (x < NX) && (y < NY);
@@ -600,6 +600,25 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     }
 }
 
+// Launch shapes: tile width TX (x cells per 256-thread block, TY = 256/TX rows),
+// XCD-contiguous tile order (SWZ), and a grid-stride persistent launch (PERSIST)
+// whose grid fills the chip once.  The autotuner picks among them per tunnel.
+template <int S, bool EXACT, int TX, bool SWZ>
+__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+    static_assert(TX >= 64 && TX <= kBlock && kBlock % TX == 0, "whole wavefronts per row");
+    step_tile<S, EXACT, TX>(a, SWZ ? xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x);
+}
+// Persistent form: kPersistBlocksPerCU resident blocks per CU walk the tiles; the
+// register budget is capped so that many blocks really are resident.
+template <int S, bool EXACT, int TX, bool SWZ>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kPersistBlocksPerCU)))
+k_step_persist(StepArgs a) {
+    static_assert(TX >= 64 && TX <= kBlock && kBlock % TX == 0, "whole wavefronts per row");
+    // gridDim.x is a multiple of 8, so block b stays on XCD b % 8 for every tile.
+    for (uint32_t b = blockIdx.x; b < a.ntiles; b += gridDim.x)
+        step_tile<S, EXACT, TX>(a, SWZ ? xcd_swizzle(b, a.ntiles) : b);
+}
+
 // LBM_INIT (wgsl.ts:147-154) on both buffers: feq(i, 1, (uIn,0,0)) everywhere,
 // fields = (uIn,0,0,1) and the outlet velocity plane = (uIn,0,0).
 template <int S>
@@ -672,16 +691,21 @@ __global__ void __launch_bounds__(256) k_flags(const uint32_t* __restrict__ soli
     flags[idx] = fl;
 }
 
-// Fixed-order reduction of the tile slots into one TileSlot.
-__global__ void __launch_bounds__(256) k_reduce_slots(const TileSlot* __restrict__ s, uint32_t ntiles, TileSlot* out) {
+// Fixed-order reduction of the force slots: block b sums the contiguous chunk b of
+// the n slots into out[b] (strided per thread, then a fixed tree), so the result is
+// the same bit pattern on every run.
+__global__ void __launch_bounds__(256) k_reduce_slots(const TileSlot* __restrict__ s, uint32_t n, TileSlot* out) {
     __shared__ double sd[256][9];
     __shared__ long long sq[256][9];
     __shared__ unsigned long long sg[256];
     const int t = threadIdx.x;
+    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = blockIdx.x * per;
+    const uint32_t hi = lo + per < n ? lo + per : n;
     double d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     long long q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     unsigned long long g = 0;
-    for (uint32_t i = t; i < ntiles; i += 256) {
+    for (uint32_t i = lo + t; i < hi; i += 256) {
         for (int k = 0; k < 9; k++) { d[k] += s[i].d[k]; q[k] += s[i].q[k]; }
         g += s[i].guard;
     }
@@ -696,9 +720,10 @@ __global__ void __launch_bounds__(256) k_reduce_slots(const TileSlot* __restrict
         __syncthreads();
     }
     if (t == 0) {
-        for (int k = 0; k < 9; k++) { out->d[k] = sd[0][k]; out->q[k] = sq[0][k]; }
-        out->guard = sg[0];
-        out->pad = 0;
+        TileSlot* o = out + blockIdx.x;
+        for (int k = 0; k < 9; k++) { o->d[k] = sd[0][k]; o->q[k] = sq[0][k]; }
+        o->guard = sg[0];
+        o->pad = 0;
     }
 }
 
@@ -733,7 +758,6 @@ void fill_step_consts(const aero_ctx* c, StepArgs* a) {
     a->plane_bytes = (uint32_t)(c->ns * (c->storage == AERO_STORE_F32 ? 4u : 2u));
     a->pop_bytes = (uint32_t)c->pop_bytes;
     a->nx = c->grid.nx; a->ny = c->grid.ny; a->nz = c->grid.nz;
-    a->tiles_x = c->tiles_x; a->tiles_y = c->tiles_y; a->ntiles = c->ntiles;
     a->u_in = c->sim.u_in;
     a->smag_c = (float)(18.0 * 1.41421356) * c->sim.smag * c->sim.smag;
     for (int i = 0; i < 19; i++) {
@@ -768,35 +792,72 @@ hipError_t launch_flags(aero_ctx* c) {
     return hipGetLastError();
 }
 
-hipError_t launch_step(aero_ctx* c, int write_fields) {
+const LaunchShape kShapes[kNumShapes] = {
+    {64, false, false}, {64, true, false}, {64, false, true}, {64, true, true},
+    {128, false, false}, {128, true, false},
+    {256, false, false}, {256, true, false}, {256, false, true}, {256, true, true},
+};
+
+void tile_geometry(const aero_ctx* c, int shape, uint32_t* tiles_x, uint32_t* tiles_y, uint32_t* ntiles) {
+    const uint32_t tx = (uint32_t)kShapes[shape].tx, ty = (uint32_t)kBlock / tx;
+    *tiles_x = (c->grid.nx + tx - 1) / tx;
+    *tiles_y = (c->grid.ny + ty - 1) / ty;
+    *ntiles = *tiles_x * *tiles_y * c->grid.nz;
+}
+
+namespace {
+template <int S, bool EXACT>
+hipError_t launch_shape(aero_ctx* c, const StepArgs& a, int shape) {
+    const LaunchShape& sh = kShapes[shape];
+    uint32_t grid = a.ntiles;
+    if (sh.persist) {
+        grid = c->persist_blocks < a.ntiles ? c->persist_blocks : a.ntiles;
+        grid = grid >= 8 ? grid & ~7u : grid;
+    }
+#define AERO_L(TX, SWZ) hipLaunchKernelGGL((k_step<S, EXACT, TX, SWZ>), dim3(grid), dim3(kBlock), 0, c->stream, a)
+#define AERO_P(TX, SWZ) hipLaunchKernelGGL((k_step_persist<S, EXACT, TX, SWZ>), dim3(grid), dim3(kBlock), 0, c->stream, a)
+    switch (shape) {
+    case 0: AERO_L(64, false); break;
+    case 1: AERO_L(64, true); break;
+    case 2: AERO_P(64, false); break;
+    case 3: AERO_P(64, true); break;
+    case 4: AERO_L(128, false); break;
+    case 5: AERO_L(128, true); break;
+    case 6: AERO_L(256, false); break;
+    case 7: AERO_L(256, true); break;
+    case 8: AERO_P(256, false); break;
+    default: AERO_P(256, true); break;
+    }
+#undef AERO_L
+#undef AERO_P
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_step_shape(aero_ctx* c, int write_fields, int shape) {
     StepArgs a{};
     fill_step_consts(c, &a);
     a.fin = c->f[c->cur];
     a.fout = c->f[c->cur ^ 1];
     a.write_fields = write_fields;
-    const dim3 grid(c->ntiles), block(kBlock);
+    tile_geometry(c, shape, &a.tiles_x, &a.tiles_y, &a.ntiles);
+    hipError_t e;
     const bool exact = c->exact != 0;
     switch (c->storage) {
-    case AERO_STORE_F32:
-        if (exact) hipLaunchKernelGGL((k_step<AERO_STORE_F32, true>), grid, block, 0, c->stream, a);
-        else hipLaunchKernelGGL((k_step<AERO_STORE_F32, false>), grid, block, 0, c->stream, a);
-        break;
-    case AERO_STORE_F16S:
-        if (exact) hipLaunchKernelGGL((k_step<AERO_STORE_F16S, true>), grid, block, 0, c->stream, a);
-        else hipLaunchKernelGGL((k_step<AERO_STORE_F16S, false>), grid, block, 0, c->stream, a);
-        break;
-    default:
-        if (exact) hipLaunchKernelGGL((k_step<AERO_STORE_BF16S, true>), grid, block, 0, c->stream, a);
-        else hipLaunchKernelGGL((k_step<AERO_STORE_BF16S, false>), grid, block, 0, c->stream, a);
-        break;
+    case AERO_STORE_F32: e = exact ? launch_shape<AERO_STORE_F32, true>(c, a, shape) : launch_shape<AERO_STORE_F32, false>(c, a, shape); break;
+    case AERO_STORE_F16S: e = exact ? launch_shape<AERO_STORE_F16S, true>(c, a, shape) : launch_shape<AERO_STORE_F16S, false>(c, a, shape); break;
+    default: e = exact ? launch_shape<AERO_STORE_BF16S, true>(c, a, shape) : launch_shape<AERO_STORE_BF16S, false>(c, a, shape); break;
     }
-    hipError_t e = hipGetLastError();
     if (e == hipSuccess) c->cur ^= 1;
     return e;
 }
 
+hipError_t launch_step(aero_ctx* c, int write_fields) { return launch_step_shape(c, write_fields, c->shape); }
+
 hipError_t launch_reduce_slots(aero_ctx* c) {
-    hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, c->slots, c->ntiles * (uint32_t)kWaves, (TileSlot*)c->red_out);
+    TileSlot* parts = (TileSlot*)c->red_out;
+    hipLaunchKernelGGL(k_reduce_slots, dim3(kRedParts), dim3(256), 0, c->stream, c->slots, c->nslots, parts);
+    hipLaunchKernelGGL(k_reduce_slots, dim3(1), dim3(256), 0, c->stream, parts, kRedParts, parts + kRedParts);
     return hipGetLastError();
 }
 

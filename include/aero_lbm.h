@@ -148,6 +148,18 @@ int aero_step_timed(aero_ctx* ctx, uint32_t nsteps, float* ms);
 int aero_profile_begin(aero_ctx* ctx, uint32_t max_calls);
 int aero_profile_end(aero_ctx* ctx, float* ms, uint64_t* steps);
 
+/* Launch-shape autotuner (no reference counterpart: the reference dispatches one
+ * 1-D grid of 128-thread workgroups, engine.ts:710).  Times each stream-collide launch
+ * shape (tile width 64/128/256 cells, XCD-ordered tiles, persistent grid) for
+ * steps_per_shape steps on a copy of the current state, restores the state, and
+ * keeps the fastest for this context.  Results are cached per process and key
+ * (device, nx, ny, nz, storage, math); a cached key returns at once with
+ * *us_per_step = -1.  Outputs are nullable. */
+int aero_autotune(aero_ctx* ctx, uint32_t steps_per_shape, int* shape, float* us_per_step);
+/* Selects a launch shape directly (0..9) and reports the current one. */
+int aero_set_launch_shape(aero_ctx* ctx, int shape);
+int aero_get_launch_shape(aero_ctx* ctx, int* shape, int* tile_x, int* xcd_order, int* persistent);
+
 /* nsteps semi-Lagrangian heat steps (HEAT, wgsl.ts:460-511; engine.ts:659-660,711-712). */
 int aero_heat_step(aero_ctx* ctx, const aero_heat* heat, uint32_t nsteps);
 

@@ -255,3 +255,50 @@ def test_forces_reset_and_step_count():
     e.step(3)
     f = e.read_forces(reset=True)
     assert f is not None and f.frontWing.drag != 0
+
+
+@pytest.mark.parametrize("math", ["exact", "fast"])
+def test_every_launch_shape_gives_identical_populations(math):
+    """Tile width, XCD order and persistent launches change only the schedule: the
+    populations are bit-identical and the force sums agree to FP64 rounding."""
+    rng = np.random.default_rng(12)
+    dims = (200, 22, 12)  # ragged for 64-, 128- and 256-cell tiles
+    solid, emit = mixed_geometry(dims, rng)
+    results = []
+    for shape in range(10):
+        e = engine_for(dims, u=0.06, math=math)
+        e.set_flags(solid, emit)
+        e.reset()
+        e.set_launch_shape(shape)
+        assert e.launch_shape()["shape"] == shape
+        e.step(7)
+        d, q, _ = e.read_forces_raw()
+        results.append((e.populations(), e.fields(), d, q))
+        e.close()
+    p0, fl0, d0, q0 = results[0]
+    for shape, (p, fl, d, q) in enumerate(results):
+        assert np.array_equal(p, p0), shape
+        assert np.array_equal(fl, fl0), shape
+        assert np.array_equal(q, q0), shape
+        assert np.abs(d - d0).max() <= 1e-12 * np.abs(d0).max(), shape
+
+
+def test_autotune_restores_state_and_caches():
+    rng = np.random.default_rng(13)
+    dims = (200, 22, 12)
+    solid, emit = mixed_geometry(dims, rng)
+    e = engine_for(dims, u=0.06, math="fast")
+    e.set_flags(solid, emit)
+    e.reset()
+    e.step(5)
+    before = e.populations()
+    d0, q0, n0 = e.read_forces_raw()
+    shape, us = e.autotune(steps_per_shape=3)
+    assert 0 <= shape < 10 and us > 0
+    assert e.launch_shape()["shape"] == shape
+    assert np.array_equal(e.populations(), before)
+    d1, q1, n1 = e.read_forces_raw()
+    assert n1 == n0 and np.array_equal(q1, q0) and np.array_equal(d1, d0)
+    e2 = engine_for(dims, u=0.06, math="fast")
+    shape2, us2 = e2.autotune(steps_per_shape=3)
+    assert shape2 == shape and us2 == -1.0  # cached for the same device, dims and storage
