@@ -120,6 +120,21 @@ def main() -> None:
     d, q, nsteps = e.read_forces_raw(reset=True)
     guard = e.guard_count()
 
+    # HBM bytes per launch of this kernel and shape from the calibrated FETCH_SIZE /
+    # WRITE_SIZE counter runs recorded in profiles/hbm_traffic.json (null if none).
+    traffic = None
+    try:
+        prof = json.loads((ROOT / "profiles" / "hbm_traffic.json").read_text())
+        st = {"f32": 0, "f16s": 1, "bf16s": 2}[args.storage]
+        kname = (f"k_step{'_persist' if launch['persistent'] else ''}<{st}, "
+                 f"{'true' if args.math == 'exact' else 'false'}, {launch['tile_x']}, "
+                 f"{'true' if launch['xcd_order'] else 'false'}>")
+        rec = prof.get(f"{wl.name}_{wl.dims[0]}x{wl.dims[1]}x{wl.dims[2]}", {}).get(kname)
+        if rec:
+            traffic = rec["fetch_bytes"] + rec["write_bytes"]
+    except (OSError, ValueError, KeyError):
+        traffic = None
+
     mlups = wl.cells * args.steps / wall / 1e6
     bpu = bytes_per_update(args.storage)
     launch_ms = lbm_ms / max(lbm_steps, 1)
@@ -152,7 +167,7 @@ def main() -> None:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "kernel": f"k_step<{args.storage},{args.math}>",
             "bytes_per_cell": bpu,
             "launch_us": round(launch_ms * 1e3, 2),
