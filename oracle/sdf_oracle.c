@@ -10,6 +10,9 @@
  *   src/engine/wgsl.ts:294-322  VOXELIZE.cs_main
  *   src/engine/f1.ts:37-46      rakeAngle/toBodyFrame (rake rotation about the front axle)
  *
+ *   src/engine/f1.ts:20-320     analytic F1 car (sdF1 and its parts), kind 0
+ *   src/engine/f1.ts:397-407    f1Emission
+ *
  * Kind 16 ("primitive scene") is this build's own extension: a union of the
  * reference's primitives with per-primitive material ids, evaluated in the world
  * frame or in the raked body frame, plus point heat emitters.  It describes the
@@ -118,6 +121,329 @@ static float sdCylZ(f3 p, float r, float hz) { /* f1.ts:60-63 */
 }
 typedef struct { float d, m; } res2;
 static res2 opU(res2 a, res2 b) { return (b.d < a.d) ? b : a; } /* wgsl.ts:76 */
+/* wgsl.ts:46-54 */
+static float sdTaperC(f3 p, float x0, float x1, f2 c0, f2 c1, f2 s0, f2 s1, float r0, float r1) {
+    float t = clampf_((p.x - x0) / (x1 - x0), 0.0f, 1.0f);
+    f2 c = v2(mixf(c0.x, c1.x, t), mixf(c0.y, c1.y, t));
+    f2 s = v2(mixf(s0.x, s1.x, t), mixf(s0.y, s1.y, t));
+    float r = mn(mixf(r0, r1, t), mn(s.x, s.y));
+    float d2 = sdRoundRect(sub2(v2(p.y, p.z), c), s, r);
+    float dx = fabsf(p.x - 0.5f * (x0 + x1)) - 0.5f * (x1 - x0);
+    return opExtrude(d2, dx) * 0.8f;
+}
+
+/* ---- analytic F1 car (kind 0), src/engine/f1.ts:20-320,397-407 ---- */
+#define DEG 0.017453292f
+#define FA 1.0f
+#define RA 4.6f
+#define WR 0.36f
+#define FWZ 0.84f
+#define FHW 0.152f
+#define RWZ 0.80f
+#define RHW 0.20f
+#define RA26 4.4f
+#define WR26 0.355f
+#define FHW26 0.14f
+#define RHW26 0.1875f
+#define BODY_X 0.9444444f
+#define BODY_Z 0.96f
+#define BODY_LIP 0.96f
+
+static const og_shape* g_sh; /* the `shape` uniform of the current voxelize call */
+
+static f3 add3(f3 a, f3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static float smoothstepf(float e0, float e1, float x) {
+    float t = clampf_((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+/* f1.ts:37-46 */
+static float rakeAngle(void) { return atanf((g_sh->rearRide - g_sh->rideH) / (RA26 - FA)); }
+static f3 toBodyFrame(f3 pw) {
+    float th = rakeAngle();
+    float c = cosf(th), s = sinf(th);
+    float dx = pw.x - FA, dy = pw.y - g_sh->rideH;
+    f3 p = v3(FA + dx * c + dy * s, g_sh->rideH - dx * s + dy * c, pw.z);
+    float x = p.x;
+    if (x > FA) x = (x > RA26) ? x + (RA - RA26) : FA + (x - FA) / BODY_X;
+    return v3(x, p.y, p.z / BODY_Z);
+}
+/* f1.ts:50-59 */
+static float smin(float a, float b, float k) {
+    float h = mx(k - fabsf(a - b), 0.0f) / k;
+    return mn(a, b) - h * h * k * 0.25f;
+}
+static float smax(float a, float b, float k) { return -smin(-a, -b, k); }
+static float sdEllipsoid(f3 p, f3 r) {
+    float k0 = len3(v3(p.x / r.x, p.y / r.y, p.z / r.z));
+    float k1 = len3(v3(p.x / (r.x * r.x), p.y / (r.y * r.y), p.z / (r.z * r.z)));
+    return k0 * (k0 - 1.0f) / mx(k1, 1e-5f);
+}
+/* f1.ts:65-68 */
+static float sdWingZ(f3 p, f2 le, float chord, float ang, float thick, float z0, float z1, float rise, float taper,
+                     float curl) {
+    float s = smoothstepf(curl, z1, p.z);
+    return sdWing(p, v2(le.x, le.y + rise * s), chord * (1.0f - taper * s), ang + rise * s * 0.6f, thick, z0, z1) *
+           0.85f;
+}
+/* f1.ts:70-74 */
+static float tunnelRoof(float x, float b) {
+    if (x < 2.5f) return mixf(b + 0.12f, b + 0.05f, smoothstepf(1.55f, 2.5f, x));
+    if (x < 3.8f) return b + 0.05f;
+    return mixf(b + 0.05f, b + 0.25f, smoothstepf(3.8f, 4.98f, x));
+}
+/* f1.ts:76-96 */
+static float sdFloor(f3 p, float b) {
+    float roof = tunnelRoof(p.x, b);
+    float zLim = mixf(0.80f, 0.58f, smoothstepf(3.6f, 4.15f, p.x));
+    float top = (fabsf(p.z - 0.40f) < 0.28f) ? roof + 0.03f : b + 0.05f;
+    float slab = mx(mx(b - p.y, p.y - top), mx(fabsf(p.x - 3.265f) - 1.715f, p.z - zLim));
+    float tunnel = mx(p.y - roof, fabsf(p.z - 0.40f) - 0.255f);
+    float d = mx(slab, -tunnel) * 0.8f;
+    for (int i = 0; i < 2; i++) {
+        float fz = 0.26f + (float)i * 0.16f;
+        float fence = sdBox(sub3(p, v3(1.84f, b + 0.08f, fz)), v3(0.28f, 0.08f, 0.012f));
+        d = mn(d, mx(fence, p.y - roof - 0.02f));
+    }
+    for (int i = 0; i < 2; i++) {
+        float sz = 0.30f + (float)i * 0.20f;
+        float strake = sdBox(sub3(p, v3(4.62f, b + 0.17f, sz)), v3(0.34f, 0.17f, 0.008f));
+        d = mn(d, mx(strake, p.y - roof));
+    }
+    d = mn(d, sdCapsule(p, v3(2.25f, b + 0.065f, 0.79f), v3(3.85f, b + 0.05f, 0.78f), 0.02f));
+    return d;
+}
+/* f1.ts:98-107 */
+static float sdTyre(f3 p, f3 c, float hw) {
+    f3 q = sub3(p, c);
+    return sdRoundRect(v2(len2(v2(q.x, q.y)) - 0.2925f, q.z), v2(0.0625f, hw), 0.06f);
+}
+static float sdWheelCover(f3 p, f3 c, float hw) {
+    f3 q = sub3(p, c);
+    float d = sdCylZ(q, 0.234f, hw - 0.028f);
+    d = mn(d, sdCylZ(sub3(q, v3(0.0f, 0.0f, hw - 0.02f)), 0.045f, 0.02f));
+    return d;
+}
+/* f1.ts:109-118 */
+static res2 sdWheelSet(f3 p, res2 res0, f3 c, float hw, float inboard) {
+    res2 res = res0;
+    if (sdBox(sub3(p, c), v3(WR26 + 0.02f, WR26 + 0.02f, hw + 0.2f)) > res.d) return res;
+    res2 t = {sdTyre(p, c, hw), 2.0f};
+    res = opU(res, t);
+    res2 w = {sdWheelCover(p, c, hw), 9.0f};
+    res = opU(res, w);
+    float duct = sdCylZ(sub3(p, v3(c.x, c.y, c.z - hw - inboard * 0.5f)), 0.17f, inboard * 0.5f);
+    float ductWing = sdRoundBox(sub3(p, v3(c.x - 0.04f, c.y + 0.21f, c.z - hw - 0.05f)), v3(0.16f, 0.012f, 0.05f), 0.006f);
+    res2 dd = {mn(duct, ductWing), 10.0f};
+    res = opU(res, dd);
+    return res;
+}
+/* f1.ts:120-133 */
+static float sdSuspension(f3 p, float b) {
+    float d = sdCapsule(p, v3(1.10f, b + 0.43f, 0.17f), v3(FA + 0.02f, WR + 0.16f, FWZ - FHW - 0.05f), 0.017f);
+    d = mn(d, sdCapsule(p, v3(1.42f, b + 0.40f, 0.17f), v3(FA + 0.02f, WR + 0.16f, FWZ - FHW - 0.05f), 0.017f));
+    d = mn(d, sdCapsule(p, v3(0.98f, b + 0.19f, 0.15f), v3(FA, WR - 0.13f, FWZ - FHW - 0.05f), 0.017f));
+    d = mn(d, sdCapsule(p, v3(1.38f, b + 0.17f, 0.15f), v3(FA, WR - 0.13f, FWZ - FHW - 0.05f), 0.017f));
+    d = mn(d, sdCapsule(p, v3(FA + 0.03f, WR - 0.10f, FWZ - FHW - 0.07f), v3(1.20f, b + 0.47f, 0.16f), 0.014f));
+    d = mn(d, sdCapsule(p, v3(0.86f, b + 0.29f, 0.16f), v3(0.88f, WR + 0.02f, FWZ - FHW - 0.05f), 0.013f));
+    d = mn(d, sdCapsule(p, v3(4.30f, b + 0.42f, 0.20f), v3(RA, WR + 0.17f, RWZ - RHW - 0.05f), 0.017f));
+    d = mn(d, sdCapsule(p, v3(4.78f, b + 0.40f, 0.16f), v3(RA, WR + 0.17f, RWZ - RHW - 0.05f), 0.017f));
+    d = mn(d, sdCapsule(p, v3(4.28f, b + 0.16f, 0.24f), v3(RA, WR - 0.14f, RWZ - RHW - 0.05f), 0.017f));
+    d = mn(d, sdCapsule(p, v3(4.82f, b + 0.15f, 0.18f), v3(RA, WR - 0.14f, RWZ - RHW - 0.05f), 0.017f));
+    d = mn(d, sdCapsule(p, v3(RA - 0.03f, WR + 0.14f, RWZ - RHW - 0.07f), v3(4.40f, b + 0.18f, 0.22f), 0.014f));
+    return d;
+}
+/* f1.ts:135-152 */
+static uint32_t variant(float v) { return (uint32_t)(v + 0.5f); }
+static float noseTipY(float b) { return b + (variant(g_sh->nose) == 2u ? 0.25f : 0.17f); }
+static float noseStartX(void) { return variant(g_sh->nose) == 1u ? 0.34f : 0.10f; }
+static float sdNose(f3 p, float b) {
+    uint32_t v = variant(g_sh->nose);
+    float x0 = noseStartX();
+    float y0 = noseTipY(b);
+    f2 s0 = v2(0.055f, 0.11f);
+    if (v == 1u) s0 = v2(0.075f, 0.15f);
+    if (v == 2u) s0 = v2(0.045f, 0.075f);
+    float d = sdTaperC(p, x0, 1.26f, v2(y0, 0.0f), v2(b + 0.36f, 0.0f), s0, v2(0.15f, 0.17f), 0.04f, 0.12f);
+    d = smin(d, sdEllipsoid(sub3(p, v3(x0 + 0.02f, y0, 0.0f)), v3(0.09f, s0.x, s0.y)), 0.04f);
+    if (v != 0u) {
+        float yw = 0.19f + b * 0.45f;
+        float pylon = sdRoundBox(sub3(p, v3(x0 + 0.06f, 0.5f * (yw + y0), 0.09f)),
+                                 v3(0.09f, 0.5f * (y0 - yw) + 0.02f, 0.006f), 0.004f);
+        d = mn(d, pylon);
+    }
+    return d;
+}
+/* f1.ts:154-159 */
+static float sdEngineCover(f3 p, float b) {
+    uint32_t v = variant(g_sh->cover);
+    if (v == 1u) return sdTaperC(p, 2.96f, 4.86f, v2(b + 0.60f, 0.0f), v2(b + 0.40f, 0.0f), v2(0.32f, 0.31f), v2(0.12f, 0.13f), 0.15f, 0.08f);
+    if (v == 2u) return sdTaperC(p, 2.96f, 4.70f, v2(b + 0.57f, 0.0f), v2(b + 0.34f, 0.0f), v2(0.29f, 0.19f), v2(0.06f, 0.05f), 0.12f, 0.04f);
+    return sdTaperC(p, 2.96f, 4.86f, v2(b + 0.58f, 0.0f), v2(b + 0.36f, 0.0f), v2(0.30f, 0.24f), v2(0.08f, 0.07f), 0.14f, 0.06f);
+}
+/* f1.ts:161-188 */
+static float sdSidepods(f3 p, float b) {
+    uint32_t v = variant(g_sh->sidepod);
+    float pod;
+    if (v == 1u) {
+        pod = sdTaperC(p, 2.30f, 4.05f, v2(b + 0.40f, 0.36f), v2(b + 0.24f, 0.24f), v2(0.16f, 0.08f), v2(0.07f, 0.05f), 0.06f, 0.04f);
+        pod = smax(pod, -sdRoundBox(sub3(p, v3(2.27f, b + 0.42f, 0.38f)), v3(0.06f, 0.12f, 0.05f), 0.03f), 0.02f);
+        return pod;
+    }
+    if (v == 2u) {
+        pod = sdTaperC(p, 2.30f, 4.25f, v2(b + 0.41f, 0.52f), v2(b + 0.24f, 0.30f), v2(0.18f, 0.28f), v2(0.08f, 0.11f), 0.10f, 0.05f);
+        pod = smax(pod, -sdCapsule(p, v3(2.30f, b + 0.13f, 0.84f), v3(3.70f, b + 0.08f, 0.58f), 0.15f), 0.05f);
+        pod = smax(pod, -sdCapsule(p, v3(2.55f, b + 0.66f, 0.52f), v3(4.00f, b + 0.40f, 0.36f), 0.11f), 0.04f);
+        pod = smax(pod, -sdRoundBox(sub3(p, v3(2.28f, b + 0.47f, 0.57f)), v3(0.06f, 0.10f, 0.17f), 0.05f), 0.02f);
+        return pod;
+    }
+    if (v == 3u) {
+        pod = sdTaperC(p, 2.30f, 4.25f, v2(b + 0.47f, 0.52f), v2(b + 0.26f, 0.28f), v2(0.15f, 0.25f), v2(0.07f, 0.10f), 0.12f, 0.05f);
+        pod = smax(pod, -sdCapsule(p, v3(2.25f, b + 0.17f, 0.86f), v3(3.70f, b + 0.10f, 0.56f), 0.22f), 0.05f);
+        pod = smax(pod, -sdRoundBox(sub3(p, v3(2.28f, b + 0.52f, 0.55f)), v3(0.06f, 0.07f, 0.17f), 0.04f), 0.02f);
+        return pod;
+    }
+    pod = sdTaperC(p, 2.30f, 4.25f, v2(b + 0.41f, 0.52f), v2(b + 0.22f, 0.28f), v2(0.18f, 0.25f), v2(0.07f, 0.10f), 0.12f, 0.05f);
+    pod = smax(pod, -sdCapsule(p, v3(2.30f, b + 0.13f, 0.82f), v3(3.70f, b + 0.08f, 0.56f), 0.16f), 0.05f);
+    pod = smax(pod, -sdRoundBox(sub3(p, v3(2.28f, b + 0.47f, 0.57f)), v3(0.06f, 0.10f, 0.17f), 0.05f), 0.02f);
+    return pod;
+}
+/* f1.ts:190-205 */
+static float sdBodywork(f3 p, float b) {
+    float d = sdNose(p, b);
+    d = smin(d, sdTaperC(p, 1.20f, 2.24f, v2(b + 0.36f, 0.0f), v2(b + 0.42f, 0.0f), v2(0.15f, 0.17f), v2(0.25f, 0.29f), 0.12f, 0.16f), 0.08f);
+    d = smin(d, sdTaperC(p, 2.20f, 3.06f, v2(b + 0.42f, 0.0f), v2(b + 0.42f, 0.0f), v2(0.25f, 0.30f), v2(0.25f, 0.34f), 0.16f, 0.16f), 0.06f);
+    d = smin(d, sdEngineCover(p, b), 0.10f);
+    float keel = sdTaperC(p, 1.30f, 4.70f, v2(b + 0.22f, 0.0f), v2(b + 0.20f, 0.0f), v2(0.19f, 0.19f), v2(0.17f, 0.14f), 0.06f, 0.05f);
+    d = smin(d, keel, 0.08f);
+    d = smin(d, sdSidepods(p, b), 0.12f);
+    float airbox = sdRoundBox(sub3(p, v3(3.02f, b + 0.84f, 0.0f)), v3(0.17f, 0.11f, 0.12f), 0.08f);
+    airbox = smax(airbox, -sdEllipsoid(sub3(p, v3(2.85f, b + 0.86f, 0.0f)), v3(0.09f, 0.066f, 0.078f)), 0.015f);
+    d = smin(d, airbox, 0.10f);
+    d = smax(d, -sdRoundBox(sub3(p, v3(2.62f, b + 0.72f, 0.0f)), v3(0.42f, 0.13f, 0.215f), 0.10f), 0.03f);
+    return d;
+}
+/* f1.ts:207-213 */
+static float sdHalo(f3 p, float b) {
+    float d = sdCapsule(p, v3(2.17f, b + 0.62f, 0.0f), v3(2.36f, b + 0.90f, 0.0f), 0.034f);
+    d = smin(d, sdCapsule(p, v3(2.36f, b + 0.90f, 0.0f), v3(2.52f, b + 0.92f, 0.20f), 0.031f), 0.03f);
+    d = smin(d, sdCapsule(p, v3(2.52f, b + 0.92f, 0.20f), v3(2.82f, b + 0.88f, 0.26f), 0.031f), 0.03f);
+    d = smin(d, sdCapsule(p, v3(2.82f, b + 0.88f, 0.26f), v3(3.02f, b + 0.74f, 0.25f), 0.033f), 0.03f);
+    return d;
+}
+/* f1.ts:225-255 */
+static float sdFrontWing(f3 p, float yw) {
+    float ff = g_sh->frontFlap;
+    uint32_t v = variant(g_sh->frontWing);
+    float d = 1e3f;
+    float ep = 0.0f;
+    if (v == 1u) {
+        d = sdWingZ(p, v2(-0.03f, yw), 0.31f, 2.0f * DEG + ff * 0.1f, 0.034f, 0.0f, 0.94f, 0.03f, 0.2f, 0.5f);
+        d = mn(d, sdWingZ(p, v2(0.21f, yw + 0.036f), 0.17f, 9.0f * DEG + ff * 0.45f, 0.024f, 0.0f, 0.94f, 0.15f, 0.35f, 0.3f));
+        d = mn(d, sdWingZ(p, v2(0.335f, yw + 0.075f + ff * 0.03f), 0.14f, 17.0f * DEG + ff * 0.75f, 0.020f, 0.10f, 0.94f, 0.19f, 0.35f, 0.25f));
+        d = mn(d, sdWingZ(p, v2(0.425f, yw + 0.118f + ff * 0.07f), 0.12f, 26.0f * DEG + ff, 0.018f, 0.12f, 0.94f, 0.22f, 0.4f, 0.25f));
+        ep = sdRoundRect(v2(p.x - 0.25f, p.y - (yw + 0.17f)), v2(0.29f, 0.20f), 0.08f);
+    } else if (v == 2u) {
+        d = sdWingZ(p, v2(-0.03f, yw), 0.31f, 3.0f * DEG + ff * 0.1f, 0.034f, 0.0f, 0.94f, -0.02f, 0.15f, 0.55f);
+        d = mn(d, sdWingZ(p, v2(0.21f, yw + 0.04f), 0.19f, 13.0f * DEG + ff * 0.45f, 0.024f, 0.0f, 0.94f, -0.04f, 0.35f, 0.35f));
+        d = mn(d, sdWingZ(p, v2(0.35f, yw + 0.085f + ff * 0.03f), 0.16f, 22.0f * DEG + ff * 0.75f, 0.020f, 0.06f, 0.94f, -0.06f, 0.4f, 0.35f));
+        d = mn(d, sdWingZ(p, v2(0.45f, yw + 0.13f + ff * 0.07f), 0.13f, 31.0f * DEG + ff, 0.018f, 0.08f, 0.94f, -0.08f, 0.45f, 0.35f));
+        ep = sdRoundRect(v2(p.x - 0.25f, p.y - (yw + 0.06f)), v2(0.29f, 0.10f), 0.06f);
+    } else if (v == 3u) {
+        d = sdWingZ(p, v2(-0.03f, yw), 0.30f, 1.0f * DEG + ff * 0.05f, 0.030f, 0.0f, 0.94f, 0.0f, 0.1f, 0.6f);
+        d = mn(d, sdWingZ(p, v2(0.20f, yw + 0.03f), 0.16f, 4.0f * DEG + ff * 0.3f, 0.022f, 0.0f, 0.94f, 0.02f, 0.2f, 0.5f));
+        ep = sdRoundRect(v2(p.x - 0.22f, p.y - (yw + 0.05f)), v2(0.26f, 0.085f), 0.05f);
+    } else {
+        d = sdWingZ(p, v2(-0.03f, yw), 0.31f, 2.0f * DEG + ff * 0.1f, 0.034f, 0.0f, 0.94f, 0.06f, 0.2f, 0.5f);
+        d = mn(d, sdWingZ(p, v2(0.21f, yw + 0.036f), 0.17f, 9.0f * DEG + ff * 0.45f, 0.024f, 0.0f, 0.94f, 0.09f, 0.25f, 0.45f));
+        d = mn(d, sdWingZ(p, v2(0.335f, yw + 0.075f + ff * 0.03f), 0.14f, 17.0f * DEG + ff * 0.75f, 0.020f, 0.10f, 0.94f, 0.11f, 0.25f, 0.4f));
+        d = mn(d, sdWingZ(p, v2(0.425f, yw + 0.118f + ff * 0.07f), 0.12f, 26.0f * DEG + ff, 0.018f, 0.12f, 0.94f, 0.13f, 0.25f, 0.4f));
+        ep = sdRoundRect(v2(p.x - 0.25f, p.y - (yw + 0.085f)), v2(0.29f, 0.115f), 0.07f);
+    }
+    float span = 0.94f; /* frontWingSpan() */
+    d = mn(d, mx(ep, fabsf(p.z - (span + 0.012f)) - 0.010f));
+    d = mn(d, sdRoundBox(sub3(p, v3(0.22f, yw - 0.008f, span - 0.04f)), v3(0.30f, 0.006f, 0.06f), 0.004f));
+    return d;
+}
+/* f1.ts:257-281 */
+static float sdRearWing(f3 p, float b, float yr) {
+    uint32_t v = variant(g_sh->rearWing);
+    float chordM = 0.36f, angM = g_sh->rearMain, riseM = -0.10f;
+    float chordF = 0.26f, angF = g_sh->rearFlap;
+    f2 epH = v2(0.33f, 0.29f);
+    if (v == 1u) {
+        chordM = 0.42f; angM += 3.0f * DEG; riseM = -0.15f;
+        chordF = 0.32f; angF += 6.0f * DEG;
+        epH = v2(0.36f, 0.33f);
+    } else if (v == 2u) {
+        chordM = 0.27f; angM -= 5.0f * DEG; riseM = 0.02f;
+        chordF = 0.15f; angF *= 0.55f;
+        epH = v2(0.30f, 0.25f);
+    }
+    float d = sdWingZ(p, v2(4.98f, yr), chordM, angM, 0.05f, 0.0f, 0.50f, riseM, 0.12f, 0.22f);
+    float k = chordM - 0.01f;
+    f2 te = v2(4.98f + cosf(angM) * k + -0.03f, yr + sinf(angM) * k + 0.035f);
+    d = mn(d, sdWingZ(p, te, chordF, angF, 0.03f, 0.0f, 0.49f, riseM * 0.8f, 0.10f, 0.28f));
+    d = mn(d, sdCapsule(p, v3(5.26f, yr + 0.05f, 0.0f), v3(5.30f, yr + 0.12f, 0.0f), 0.024f));
+    float ep = sdRoundRect(v2(p.x - 5.25f, p.y - (yr - 0.03f)), epH, 0.13f);
+    d = mn(d, mx(ep, fabsf(p.z - 0.513f) - 0.012f));
+    d = mn(d, sdRoundBox(sub3(p, v3(5.09f, 0.5f * (b + 0.36f + yr + 0.06f), 0.06f)),
+                         v3(0.07f, 0.5f * (yr + 0.06f - b - 0.36f), 0.008f), 0.006f));
+    return d;
+}
+/* f1.ts:283-320 */
+static res2 sdF1(f3 pw) {
+    f3 pworld = v3(pw.x, pw.y, fabsf(pw.z));
+    f3 p = toBodyFrame(pworld);
+    float b = g_sh->rideH;
+    float yw = 0.19f + b * 0.45f;
+    float yr = b + 0.80f;
+    res2 res = {1e3f, 0.0f};
+    if (sdBox(sub3(p, v3(2.72f, b + 0.52f, 0.0f)), v3(2.74f, 0.62f, 0.95f)) < res.d) {
+        res2 t;
+        t.d = sdBodywork(p, b); t.m = 0.0f; res = opU(res, t);
+        t.d = sdEllipsoid(sub3(p, v3(2.67f, b + 0.71f, 0.0f)), v3(0.15f, 0.13f, 0.12f)); t.m = 3.0f; res = opU(res, t);
+        t.d = sdHalo(p, b); t.m = 4.0f; res = opU(res, t);
+        float mirror = sdRoundBox(sub3(p, v3(2.32f, b + 0.67f, 0.60f)), v3(0.035f, 0.035f, 0.085f), 0.02f);
+        mirror = mn(mirror, sdCapsule(p, v3(2.36f, b + 0.66f, 0.53f), v3(2.45f, b + 0.50f, 0.46f), 0.011f));
+        t.d = mirror; t.m = 0.0f; res = opU(res, t);
+        float carbon = sdSuspension(p, b);
+        carbon = mn(carbon, sdTaperC(p, 4.78f, 5.40f, v2(b + 0.33f, 0.0f), v2(b + 0.33f, 0.0f), v2(0.07f, 0.065f), v2(0.045f, 0.045f), 0.04f, 0.03f));
+        carbon = mn(carbon, sdRoundBox(sub3(p, v3(3.95f, b + 0.74f, 0.0f)), v3(0.55f, 0.10f, 0.008f), 0.006f));
+        t.d = carbon; t.m = 1.0f; res = opU(res, t);
+        t.d = sdRoundBox(sub3(p, v3(5.405f, b + 0.33f, 0.0f)), v3(0.012f, 0.028f, 0.05f), 0.006f); t.m = 11.0f; res = opU(res, t);
+        float exhaust = sdCapsule(p, v3(4.95f, b + 0.44f, 0.0f), v3(5.13f, b + 0.45f, 0.0f), 0.042f);
+        exhaust = mx(exhaust, -sdCapsule(p, v3(5.0f, b + 0.44f, 0.0f), v3(5.2f, b + 0.45f, 0.0f), 0.028f));
+        t.d = exhaust; t.m = 10.0f; res = opU(res, t);
+    }
+    if (sdBox(sub3(p, v3(3.26f, b + 0.2f, 0.4f)), v3(1.75f, 0.22f, 0.42f)) < res.d) {
+        res2 t = {sdFloor(p, b), 1.0f};
+        res = opU(res, t);
+    }
+    res.d *= BODY_LIP * BODY_X;
+    if (sdBox(sub3(p, v3(0.24f, yw + 0.17f, 0.48f)), v3(0.36f, 0.34f, 0.50f)) < res.d / (BODY_LIP * BODY_X)) {
+        res2 t = {sdFrontWing(p, yw) * BODY_LIP * BODY_X, 7.0f};
+        res = opU(res, t);
+    }
+    if (sdBox(sub3(p, v3(5.20f, b + 0.64f, 0.27f)), v3(0.46f, 0.56f, 0.29f)) < res.d / (BODY_LIP * BODY_X)) {
+        res2 t = {sdRearWing(p, b, yr) * BODY_LIP * BODY_X, 8.0f};
+        res = opU(res, t);
+    }
+    res = sdWheelSet(pworld, res, v3(FA, WR26, FWZ * BODY_Z), FHW26, 0.10f);
+    res = sdWheelSet(pworld, res, v3(RA26, WR26, RWZ * BODY_Z), RHW26, 0.08f);
+    return res;
+}
+/* f1.ts:397-407 */
+static float f1Emission(f3 pw) {
+    f3 p = toBodyFrame(v3(pw.x, pw.y, fabsf(pw.z)));
+    float b = g_sh->rideH;
+    float e = 0.0f;
+    if (len2(sub2(v2(p.x, p.y), v2(FA, WR))) < 0.21f && fabsf(p.z - (FWZ - FHW - 0.03f)) < 0.12f) e = mx(e, g_sh->heatBrakes);
+    if (len2(sub2(v2(p.x, p.y), v2(RA, WR))) < 0.21f && fabsf(p.z - (RWZ - RHW - 0.03f)) < 0.12f) e = mx(e, g_sh->heatBrakes * 0.8f);
+    if (p.x > 5.02f && sdCapsule(p, v3(4.95f, b + 0.44f, 0.0f), v3(5.16f, b + 0.45f, 0.0f), 0.09f) < 0.0f) e = mx(e, g_sh->heatEngine);
+    if (p.x > 3.85f && p.x < 4.5f && p.z > 0.10f && p.y > b + 0.12f && p.y < b + 0.45f) e = mx(e, g_sh->heatRad);
+    if (p.x > 3.55f && p.x < 4.25f && p.z < 0.13f && p.y > b + 0.42f) e = mx(e, g_sh->heatRad * 1.4f);
+    return e;
+}
 
 /* ---- custom profile, wgsl.ts:79-100 ---- */
 static float profileAt(const og_shape* sh, const float* prof, int ix, int iy) {
@@ -146,13 +472,13 @@ static float sdCustom(const og_shape* sh, const float* prof, f3 p) {
 /* ---- raked body frame, f1.ts:37-46 (rotation part) ---- */
 typedef struct { float c, s, rideH; } rake_t;
 static rake_t rake_of(const og_shape* sh) {
-    const float FA = 1.0f, RA26 = 4.4f;
+    /* FA = 1.0, RA26 = 4.4 (f1.ts:24,29) */
     float th = atanf((sh->rearRide - sh->rideH) / (RA26 - FA));
     rake_t r = {cosf(th), sinf(th), sh->rideH};
     return r;
 }
 static f3 rake_apply(rake_t rk, f3 pw) {
-    const float FA = 1.0f;
+    /* FA = 1.0 */
     float dx = pw.x - FA, dy = pw.y - rk.rideH;
     return v3(FA + dx * rk.c + dy * rk.s, rk.rideH - dx * rk.s + dy * rk.c, pw.z);
 }
@@ -200,7 +526,9 @@ static float scene_emission(const og_shape* sh, const og_prim* prims, uint32_t n
 /* sceneSdf, wgsl.ts:101-111 (kinds 0 and 5 are not restated here) */
 static res2 sceneSdf(const og_shape* sh, const float* prof, const og_prim* prims, uint32_t np, f3 p) {
     res2 r;
+    g_sh = sh;
     switch (sh->kind) {
+    case 0u: return sdF1(p);
     case 1u: r.d = len3(sub3(p, v3(2.6f, 1.0f, 0.0f))) - 0.6f; r.m = 6.0f; return r;
     case 2u: r.d = sdRoundBox(sub3(p, v3(2.6f, 1.0f, 0.0f)), v3(0.5f, 0.5f, 0.5f), 0.03f); r.m = 6.0f; return r;
     case 3u: r.d = sdWing(p, v2(1.6f, 1.2f), 1.8f, -sh->aoa, 0.24f, -1.25f, 1.25f); r.m = 6.0f; return r;
@@ -210,6 +538,8 @@ static res2 sceneSdf(const og_shape* sh, const float* prof, const og_prim* prims
     }
 }
 static float sceneEmission(const og_shape* sh, const og_prim* prims, uint32_t np, f3 p) {
+    g_sh = sh;
+    if (sh->kind == 0u) return f1Emission(p);
     if (sh->kind == 16u) return scene_emission(sh, prims, np, p);
     return 0.0f;
 }

@@ -302,3 +302,35 @@ def test_autotune_restores_state_and_caches():
     e2 = engine_for(dims, u=0.06, math="fast")
     shape2, us2 = e2.autotune(steps_per_shape=3)
     assert shape2 == shape and us2 == -1.0  # cached for the same device, dims and storage
+
+
+F1_BAND = 2e-5  # metres: device vs host cosf/sinf of the per-point wing angles (f1.ts:65-68)
+
+
+@pytest.mark.parametrize("variant", [dict(), dict(front_wing=1, rear_wing=1, nose=1, sidepod=2, cover=1),
+                                     dict(front_wing=2, rear_wing=2, nose=2, sidepod=3, cover=2, ride_h=0.14,
+                                          rear_ride=0.22), dict(front_wing=3, sidepod=1)])
+def test_voxelize_reference_f1_car(variant):
+    """Kind 0: the reference's analytic F1 car.  Flags are bit-exact except for cells
+    whose oracle SDF lies within F1_BAND of the 0.3h threshold."""
+    dims = (232, 68, 52)
+    h = 10.4 / 232
+    e = engine_for(dims, h=h, origin=(-2.4, 0.0, 0.0))
+    p = aerolbm.ShapeParams(kind=aerolbm.ShapeKind.F1, rideH=variant.get("ride_h", 0.10),
+                            rearRideH=variant.get("rear_ride", 0.12), frontWing=variant.get("front_wing", 0),
+                            rearWing=variant.get("rear_wing", 0), nose=variant.get("nose", 0),
+                            sidepod=variant.get("sidepod", 0), cover=variant.get("cover", 0))
+    s = aerolbm.shape_uniform(p)
+    e.set_shape(s)
+    solid, emit = e.flags()
+    osolid, oemit, osdf = SDF.voxelize(ogrid(e), s)
+    diff = solid != osolid
+    thr = np.float32(0.3) * np.float32(h)
+    band = np.abs(osdf - thr) < F1_BAND
+    print(f"F1 {variant}: {int((osolid != 0).sum())} solid cells, {int(diff.sum())} differ (all in band: "
+          f"{bool(band[diff].all())}), band holds {int(band.sum())} cells")
+    assert band[diff].all()
+    assert diff.sum() <= max(3, int(band.sum()))
+    same = ~diff
+    assert np.array_equal(emit[same], oemit[same])
+    assert (osolid != 0).sum() > 10000

@@ -163,3 +163,50 @@ def test_custom_profile_follows_circle():
     band = np.abs(exact - thr) < 2e-3  # bilinear interpolation error of the 2 cm profile
     want = (exact < thr) & (x > 1) & (x < 98)
     assert np.array_equal((solid != 0)[~band], want[~band])
+
+
+def f1_shape(**kw):
+    s = shape(0, rideH=kw.pop("rideH", 0.10), rearRide=kw.pop("rearRide", 0.12))
+    s.frontFlap = 14 * math.pi / 180
+    s.rearMain = 9 * math.pi / 180
+    s.rearFlap = 32 * math.pi / 180
+    s.heatBrakes, s.heatEngine, s.heatRad = 520.0, 620.0, 75.0
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def test_f1_car_inside_reference_bounds_and_groups():
+    """Kind 0 (f1.ts sdF1): every solid cell centre lies inside the box that the
+    reference's shapeBounds gives for the F1 (engine.ts:505, plus the 0.3h band);
+    front-wing cells sit ahead of the front axle, rear-wing cells behind the rear
+    axle and above the body; brake heat sources surround the wheel hubs."""
+    h = 10.4 / 232
+    g = orc.grid((232, 68, 52), h, (-2.4, 0.0, 0.0))
+    s = f1_shape()
+    solid, emit, _ = SDF.voxelize(g, s)
+    x, px, py, pz = centres(g)
+    m = solid != 0
+    tol = 0.3 * h
+    assert px[m].min() >= -0.08 - tol and px[m].max() <= 5.46 + tol
+    assert py[m].max() <= 0.10 + 1.34 + tol and pz[m].max() <= 0.98 + tol
+    fw, rw = solid == 2, solid == 3
+    assert fw.sum() > 100 and rw.sum() > 100
+    # the rear-wing group includes its pylon, which starts at b + 0.36 (f1.ts:279)
+    assert px[fw].max() < 1.0 and px[rw].min() > 4.4 and py[rw].min() > 0.10 + 0.36 - tol
+    hot = emit > 0
+    assert np.isin(np.unique(emit[hot]), [520.0, 416.0, 620.0, 75.0, 105.0]).all()
+    near_front_hub = hot & (np.hypot(px - 1.0, py - 0.36) < 0.25)
+    assert near_front_hub.sum() > 0
+
+
+def test_f1_rake_lifts_the_rear():
+    h = 10.4 / 232
+    g = orc.grid((232, 68, 52), h, (-2.4, 0.0, 0.0))
+    x, px, py, pz = centres(g)
+    flat, _, _ = SDF.voxelize(g, f1_shape(rideH=0.10, rearRide=0.10))
+    raked, _, _ = SDF.voxelize(g, f1_shape(rideH=0.10, rearRide=0.25))
+    rear = px > 4.6
+    assert py[(raked != 0) & rear].mean() > py[(flat != 0) & rear].mean() + 0.02
+    front = (px > 0.2) & (px < 1.5)
+    assert abs(py[(raked != 0) & front].mean() - py[(flat != 0) & front].mean()) < 0.03
