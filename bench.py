@@ -67,6 +67,52 @@ def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
                       f"kernels (gcc -O3 -fopenmp), {dt:.1f} s"}
 
 
+def aeromap_bench(steps: int = 2000) -> dict:
+    """The 25-point ride-height map (the cube workload raked over 5x5 ride heights,
+    settle steps/2 + measured steps/2): 25 sequential FP32 runs (one engine, point
+    after point, autotuned launch shape) against one FP16-shifted batch replayed from
+    HIP graphs; plus the batch with the 8 uneven tunnels added.  Wall times cover
+    voxelize, init, stepping and force readback of every point.
+
+    Force error = |d(Fx,Fy)| / |(Fx,Fy)| of the mean measured force per point.  Over
+    steps 1001-2000 this flow is sensitive to rounding (two FP32 evaluations of the
+    algorithm differ by a few percent), so the error is also given over steps 101-400,
+    before the shear layers turn unstable."""
+    from aerolbm import aeromap
+    from aerolbm import workloads as W
+
+    pts = W.aeromap(steps, uneven=True)
+    settle, measure = steps // 2, steps - steps // 2
+    print("[bench] aero map: 25 sequential FP32 runs", file=sys.stderr, flush=True)
+    seq = aeromap.sequential(pts[:25], settle, measure, storage="f32")
+    print("[bench] aero map: batched FP16S / FP32 / FP16S + 8 uneven", file=sys.stderr, flush=True)
+    b16 = aeromap.batched(pts[:25], settle, measure, storage="f16s")
+    b32 = aeromap.batched(pts[:25], settle, measure, storage="f32")
+    b33 = aeromap.batched(pts, settle, measure, storage="f16s")
+    seq_exact = aeromap.sequential(pts[:25], settle, measure, storage="f32", math="exact")
+    print("[bench] aero map: steps 101-400, FP16S batched vs FP32", file=sys.stderr, flush=True)
+    short = W.aeromap(400, uneven=False)
+    err_det = aeromap.force_errors(aeromap.batched(short, 100, 300, storage="f16s"),
+                                   aeromap.sequential(short, 100, 300, storage="f32"))
+    err = aeromap.force_errors(b16, seq)
+    spread = aeromap.force_errors(seq, seq_exact)
+    return {
+        "points": 25, "grid": "128x64x64", "steps": steps, "measured_steps": measure,
+        "sequential_f32_s": round(seq.wall_s, 3),
+        "batched_f16s_s": round(b16.wall_s, 3),
+        "batched_f32_s": round(b32.wall_s, 3),
+        "speedup_f16s_batched_vs_sequential_f32": round(seq.wall_s / b16.wall_s, 2),
+        "batched_f16s_device_ms": round(b16.device_ms, 1),
+        "batched_f16s_mlups": round(b16.cells * steps / (b16.device_ms / 1e3) / 1e6, 1),
+        "launch_shape": b16.launch_shape,
+        "force_err_f16s_vs_f32_steps_101_400_max": float(f"{err_det.max():.3e}"),
+        "force_err_f16s_vs_f32_window_mean": float(f"{err.mean():.3e}"),
+        "force_err_f32fast_vs_f32exact_window_mean": float(f"{spread.mean():.3e}"),
+        "with_uneven": {"instances": len(pts), "cells": b33.cells, "wall_s": round(b33.wall_s, 3),
+                        "device_mlups": round(b33.cells * steps / (b33.device_ms / 1e3) / 1e6, 1)},
+    }
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +122,7 @@ def main() -> None:
     ap.add_argument("--storage", default="f32", choices=["f32", "f16s", "bf16s"])
     ap.add_argument("--math", default="fast", choices=["fast", "exact"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-aeromap", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -174,11 +221,13 @@ def main() -> None:
         },
         "forces": {"steps": nsteps, "rest_fx": d[0], "front_fy": d[4], "rear_fy": d[7], "guard_hits": guard},
     }
+    e.close()
+    if not args.no_aeromap:
+        result["aeromap"] = aeromap_bench()
     if not args.no_cpu:
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(wl)
     print(json.dumps(result))
-    e.close()
 
 
 if __name__ == "__main__":

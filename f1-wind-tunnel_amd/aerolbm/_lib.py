@@ -94,6 +94,16 @@ _PROTOS = {
     "aero_get_launch_shape": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                         C.POINTER(C.c_int)]),
     "aero_profile_end": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_uint64)]),
+    "aero_batch_create": (C.c_int, [_U32, C.POINTER(Grid), C.POINTER(Sim), C.c_int, C.POINTER(_P)]),
+    "aero_batch_destroy": (C.c_int, [_P]),
+    "aero_batch_last_error": (C.c_char_p, [_P]),
+    "aero_batch_size": (_U32, [_P]),
+    "aero_batch_instance": (_P, [_P, _U32]),
+    "aero_batch_step": (C.c_int, [_P, _U32]),
+    "aero_batch_run_graph": (C.c_int, [_P, _U32, _U32, C.POINTER(C.c_float)]),
+    "aero_batch_autotune": (C.c_int, [_P, _U32, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+    "aero_batch_set_launch_shape": (C.c_int, [_P, C.c_int]),
+    "aero_batch_get_launch_shape": (C.c_int, [_P, C.POINTER(C.c_int)]),
 }
 
 _lib = None
@@ -122,11 +132,14 @@ def exported_symbols() -> list[str]:
     return list(_PROTOS)
 
 
-def check(rc: int, ctx=None, what: str = "") -> None:
+def check(rc: int, ctx=None, what: str = "", batch=None) -> None:
     if rc == AERO_OK:
         return
     lib = load()
-    msg = (lib.aero_last_error(ctx) if ctx else lib.aero_global_error()) or b""
+    if batch:
+        msg = lib.aero_batch_last_error(batch) or b""
+    else:
+        msg = (lib.aero_last_error(ctx) if ctx else lib.aero_global_error()) or b""
     raise AeroError(f"{what}: {ERRORS.get(rc, rc)}: {msg.decode(errors='replace')}")
 
 

@@ -1,0 +1,131 @@
+"""Ride-height aero map: the 5x5 sweep of src/ui/aeromap.ts, run two ways.
+
+  sequential(points)  one engine, one map point after another (the reference's
+                      procedure, src/ui/aeromap.ts:73-114: set the heights, settle,
+                      measure, next point), FP32 populations by default;
+  batched(points)     every point (and any extra tunnels) in one AeroBatch, stepped
+                      together from a captured graph, FP16 shifted storage by default.
+
+Both voxelize each point on the GPU, start from the equilibrium fill, step
+``settle`` steps, zero the force sums, step ``measure`` more and average the forces
+over the measured steps; wall time covers all of it.  Coefficients are per unit
+dynamic pressure and refer to the full (mirrored) body:
+    Cd*A = 4 <Fx> h^2 / u^2,   Cl*A = -4 <Fy> h^2 / u^2   (positive = downforce),
+<F> being the mean per-step half-tunnel lattice force (engine.ts:752 divided by
+rho v^2 / 2).
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from .batch import AeroBatch
+from .engine import HipAeroEngine
+from .workloads import Workload
+
+
+@dataclass
+class MapPoint:
+    name: str
+    cd_a: float
+    cl_a: float
+    fx: float    # mean lattice force per step, half tunnel, all groups
+    fy: float
+    sums: np.ndarray  # FP64 force sums[9] over the measured steps
+    steps: int
+
+
+@dataclass
+class MapRun:
+    points: List[MapPoint]
+    wall_s: float
+    device_ms: Optional[float]
+    storage: str
+    cells: int
+    steps: int
+    launch_shape: Optional[int] = None
+
+    @property
+    def mlups(self) -> float:
+        return self.cells * self.steps / self.wall_s / 1e6
+
+
+def coefficients(sums: np.ndarray, steps: int, h: float, u_lb: float):
+    fx = float(sums[0] + sums[3] + sums[6]) / steps
+    fy = float(sums[1] + sums[4] + sums[7]) / steps
+    k = 4.0 * h * h / (u_lb * u_lb)
+    return k * fx, -k * fy, fx, fy
+
+
+def _point(w: Workload, sums: np.ndarray, steps: int) -> MapPoint:
+    cd_a, cl_a, fx, fy = coefficients(sums, steps, w.h, w.u_lb)
+    return MapPoint(w.name, cd_a, cl_a, fx, fy, sums.copy(), steps)
+
+
+def sequential(points: Sequence[Workload], settle: int, measure: int, storage: str = "f32",
+               math: str = "fast") -> MapRun:
+    """One engine per distinct grid, reused point after point (voxelize + reset per
+    point, as the reference's setHeights does)."""
+    t0 = time.perf_counter()
+    engines = {}
+    out = []
+    for w in points:
+        key = (w.dims, w.h, w.origin, w.u_lb, w.tau0, w.smag)
+        e = engines.get(key)
+        if e is None:
+            e = engines[key] = HipAeroEngine(storage=storage, math=math, **w.engine_kwargs())
+            e.autotune(steps_per_shape=4)
+        e.set_shape(w.shape, prims=w.prims)
+        e.reset()
+        e.step(settle)
+        e.read_forces_raw(reset=True)
+        e.step(measure)
+        d, _, n = e.read_forces_raw(reset=True)
+        out.append(_point(w, d, n))
+    for e in engines.values():
+        e.sync()
+    wall = time.perf_counter() - t0
+    for e in engines.values():
+        e.close()
+    cells = sum(int(np.prod(w.dims)) for w in points)
+    return MapRun(out, wall, None, storage, cells, settle + measure)
+
+
+def batched(points: Sequence[Workload], settle: int, measure: int, storage: str = "f16s", math: str = "fast",
+            steps_per_graph: int = 100, autotune: bool = True, launch_shape: Optional[int] = None) -> MapRun:
+    """All points in one AeroBatch; settle and measure phases replayed from graphs."""
+    t0 = time.perf_counter()
+    b = AeroBatch([w.engine_kwargs() for w in points], storage=storage, math=math)
+    try:
+        for e, w in zip(b.engines, points):
+            e.set_shape(w.shape, prims=w.prims)
+            e.reset()
+        if launch_shape is not None:
+            b.set_launch_shape(launch_shape)
+        elif autotune:
+            b.autotune(steps_per_shape=4)
+        ms = b.run_graph(settle, steps_per_graph)
+        for e in b.engines:
+            e.read_forces_raw(reset=True)
+        ms += b.run_graph(measure, steps_per_graph)
+        out = []
+        for e, w in zip(b.engines, points):
+            d, _, n = e.read_forces_raw(reset=True)
+            out.append(_point(w, d, n))
+        wall = time.perf_counter() - t0
+        shape = b.launch_shape()
+    finally:
+        b.close()
+    return MapRun(out, wall, ms, storage, b.cells, settle + measure, shape)
+
+
+def force_errors(run: MapRun, ref: MapRun) -> np.ndarray:
+    """Per point: |dF| / |F_ref| with F = (<Fx>, <Fy>), the error measure of the
+    map's two coefficients that stays defined when one of them is near zero."""
+    errs = []
+    for p, r in zip(run.points, ref.points):
+        errs.append(np.hypot(p.fx - r.fx, p.fy - r.fy) / max(np.hypot(r.fx, r.fy), 1e-30))
+    return np.array(errs)

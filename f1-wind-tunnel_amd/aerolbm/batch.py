@@ -1,0 +1,110 @@
+"""Batched tunnels: many independent tunnels stepped by one fused launch per step.
+
+The reference measures a ride-height map by moving one car through 25 settle /
+measure periods on its single engine (src/ui/aeromap.ts:73-114).  AeroBatch holds
+one ordinary engine per map point instead (``batch.engines[i]`` has the whole
+HipAeroEngine surface: set_shape, reset, read_forces, fields ...) and advances all
+of them together with aero_batch_run_graph (a captured HIP graph of fused steps)
+or aero_batch_step.  Tunnels may have different sizes and lattice parameters; they
+share one population storage mode.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import List, Optional, Sequence
+
+from . import _lib
+from ._lib import Grid, Sim, check
+from .engine import HipAeroEngine
+
+
+class AeroBatch:
+    """``specs``: one dict per tunnel with HipAeroEngine keyword arguments (dims, h,
+    origin, u_lb, tau0, smag) - a Workload's ``engine_kwargs()`` fits."""
+
+    def __init__(self, specs: Sequence[dict], storage: str = "f16s", math: str = "fast", kmh: float = 250.0):
+        if not specs:
+            raise ValueError("AeroBatch needs at least one tunnel")
+        self._lib = _lib.load()
+        n = len(specs)
+        grids = (Grid * n)()
+        sims = (Sim * n)()
+        for i, sp in enumerate(specs):
+            nx, ny, nz = sp["dims"]
+            ox, oy, oz = sp.get("origin", (-2.4, 0.0, 0.0))
+            grids[i] = Grid(nx, ny, nz, nx * ny * nz, ox, oy, oz, sp["h"])
+            sims[i] = Sim(sp.get("u_lb", 0.0925), sp.get("tau0", 0.508), sp.get("smag", 0.17), 0.0)
+        code = _lib.STORAGE_NAMES[storage] | (_lib.MATH_EXACT if math == "exact" else 0)
+        b = C.c_void_p()
+        check(self._lib.aero_batch_create(n, grids, sims, code, C.byref(b)), None, "aero_batch_create")
+        self.b = b
+        self.storage = storage
+        self.math = math
+        self.engines: List[HipAeroEngine] = []
+        for i, sp in enumerate(specs):
+            ctx = self._lib.aero_batch_instance(b, i)
+            kw = {k: sp[k] for k in ("u_lb", "tau0", "smag") if k in sp}
+            self.engines.append(HipAeroEngine._attach(C.c_void_p(ctx), self, dims=sp["dims"], h=sp["h"],
+                                                      origin=sp.get("origin", (-2.4, 0.0, 0.0)), storage=storage,
+                                                      math=math, kmh=kmh, **kw))
+        self.cells = sum(e.n for e in self.engines)
+
+    def __len__(self) -> int:
+        return len(self.engines)
+
+    def _c(self, rc: int, what: str) -> None:
+        check(rc, None, what, batch=self.b)
+
+    def close(self) -> None:
+        if getattr(self, "b", None):
+            for e in self.engines:
+                e.ctx = None
+            self._lib.aero_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, nsteps: int = 1) -> None:
+        """nsteps fused steps, launched one by one (no graph)."""
+        self._c(self._lib.aero_batch_step(self.b, int(nsteps)), "aero_batch_step")
+        for e in self.engines:
+            e.steps_since_read += int(nsteps)
+
+    def run_graph(self, nsteps: int, steps_per_graph: int = 100) -> float:
+        """nsteps fused steps replayed from a captured graph of steps_per_graph steps;
+        returns the device milliseconds of the run."""
+        ms = C.c_float()
+        self._c(self._lib.aero_batch_run_graph(self.b, int(nsteps), int(steps_per_graph), C.byref(ms)),
+                "aero_batch_run_graph")
+        for e in self.engines:
+            e.steps_since_read += int(nsteps)
+        return float(ms.value)
+
+    def autotune(self, steps_per_shape: int = 10):
+        sh = C.c_int()
+        us = C.c_float()
+        self._c(self._lib.aero_batch_autotune(self.b, int(steps_per_shape), C.byref(sh), C.byref(us)),
+                "aero_batch_autotune")
+        return int(sh.value), float(us.value)
+
+    def set_launch_shape(self, shape: int) -> None:
+        self._c(self._lib.aero_batch_set_launch_shape(self.b, int(shape)), "aero_batch_set_launch_shape")
+
+    def launch_shape(self) -> int:
+        sh = C.c_int()
+        self._c(self._lib.aero_batch_get_launch_shape(self.b, C.byref(sh)), "aero_batch_get_launch_shape")
+        return int(sh.value)
+
+    def sync(self) -> None:
+        for e in self.engines:
+            e.sync()

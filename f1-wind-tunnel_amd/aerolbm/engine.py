@@ -133,6 +133,14 @@ class HipAeroEngine:
                  h: Optional[float] = None, origin: Optional[Sequence[float]] = None,
                  u_lb: float = LATTICE_U, tau0: float = TAU0, smag: float = SMAG,
                  storage: str = "f32", math: str = "fast", kmh: float = 250.0):
+        self._setup(quality, dims, h, origin, u_lb, tau0, smag, storage, math, kmh)
+        ctx = C.c_void_p()
+        check(self._lib.aero_create(C.byref(self.grid), C.byref(self.sim), self.storage_code, C.byref(ctx)), None,
+              "aero_create")
+        self.ctx = ctx
+        self._owns = True
+
+    def _setup(self, quality, dims, h, origin, u_lb, tau0, smag, storage, math, kmh) -> None:
         nx, ny, nz = dims if dims is not None else QUALITIES[quality]
         self.dims = (int(nx), int(ny), int(nz))
         self.n = nx * ny * nz
@@ -153,16 +161,25 @@ class HipAeroEngine:
         self._lib = _lib.load()
         self.grid = Grid(nx, ny, nz, self.n, self.origin[0], self.origin[1], self.origin[2], self.h)
         self.sim = Sim(self.u_lb, self.tau0, self.smag, 0.0)
-        ctx = C.c_void_p()
-        code = _lib.STORAGE_NAMES[storage] | (_lib.MATH_EXACT if math == "exact" else 0)
-        check(self._lib.aero_create(C.byref(self.grid), C.byref(self.sim), code, C.byref(ctx)), None, "aero_create")
-        self.ctx = ctx
+        self.storage_code = _lib.STORAGE_NAMES[storage] | (_lib.MATH_EXACT if math == "exact" else 0)
         self.profile: Optional[CustomProfile] = None
+
+    @classmethod
+    def _attach(cls, ctx: C.c_void_p, owner, *, dims, h, origin, u_lb=LATTICE_U, tau0=TAU0, smag=SMAG,
+                storage="f32", math="fast", kmh=250.0) -> "HipAeroEngine":
+        """A non-owning engine over a context that another object (a batch) owns."""
+        e = cls.__new__(cls)
+        e._setup(None, dims, h, origin, u_lb, tau0, smag, storage, math, kmh)
+        e.ctx = ctx
+        e._owns = False
+        e._owner = owner
+        return e
 
     # -- lifetime --
     def close(self) -> None:
         if getattr(self, "ctx", None):
-            check(self._lib.aero_destroy(self.ctx), None, "aero_destroy")
+            if getattr(self, "_owns", False):
+                check(self._lib.aero_destroy(self.ctx), None, "aero_destroy")
             self.ctx = None
 
     def __del__(self):

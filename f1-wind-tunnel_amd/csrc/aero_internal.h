@@ -77,6 +77,15 @@ struct StepArgs {
     int write_fields;
 };
 
+// Arguments of the batched stream-collide kernel.
+struct BatchArgs {
+    const StepArgs* tab;    // per instance (fin = buffer 0, fout = buffer 1)
+    const uint32_t* begin;  // ninst + 1 tile offsets
+    uint32_t ninst;
+    int parity;             // 1: every instance reads buffer 1 and writes buffer 0
+    int write_fields;
+};
+
 }  // namespace aero
 
 struct aero_ctx {
@@ -128,6 +137,8 @@ hipError_t launch_step(aero_ctx* c, int write_fields);
 hipError_t launch_step_shape(aero_ctx* c, int write_fields, int shape);
 void tile_geometry(const aero_ctx* c, int shape, uint32_t* tiles_x, uint32_t* tiles_y, uint32_t* ntiles);
 constexpr uint32_t kRedParts = 256;
+bool batch_shape_ok(int shape);
+hipError_t launch_batch(const BatchArgs& b, int storage, int exact, int shape, uint32_t total_tiles, hipStream_t st);
 hipError_t launch_reduce_slots(aero_ctx* c);
 hipError_t launch_heat(aero_ctx* c, const aero_heat* h);
 hipError_t launch_voxelize(aero_ctx* c, const aero_shape* s, const float* d_profile,

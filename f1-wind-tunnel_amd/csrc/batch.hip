@@ -1,0 +1,329 @@
+// Batched tunnels (aero_batch_*): many independent tunnels of any sizes stepped by one
+// fused kernel launch per step, replayed from a captured HIP graph.
+//
+// The reference runs a ride-height sweep as 25 sequential settle/measure periods on
+// one tunnel (src/ui/aeromap.ts:73-114).  Here each sweep point is its own instance
+// context (created, voxelized, initialised and read through the ordinary aero_* calls
+// on aero_batch_instance()), and the batch owns only the fused stepping.
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "aero_internal.h"
+
+using namespace aero;
+
+struct aero_batch {
+    std::vector<aero_ctx*> inst;
+    int storage = 0, exact = 0, device = 0;
+    int shape = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    StepArgs* d_tab = nullptr;
+    uint32_t* d_begin = nullptr;
+    uint32_t total_tiles = 0;
+    // cached graph of `graph_steps` fused steps (even, so parity is restored)
+    hipGraphExec_t gexec = nullptr;
+    uint32_t graph_steps = 0;
+    int graph_shape = -1;
+    std::string err;
+};
+
+namespace {
+
+int bfail(aero_batch* b, int code, const std::string& m) {
+    if (b) b->err = m;
+    return code;
+}
+int bhip(aero_batch* b, hipError_t e, const char* where) {
+    return bfail(b, e == hipErrorOutOfMemory ? AERO_E_NOMEM : AERO_E_HIP,
+                 std::string(where) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
+}
+#define BCHK(b, expr)                                        \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return bhip((b), e_, #expr);   \
+    } while (0)
+
+void drop_graph(aero_batch* b) {
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+    b->graph_steps = 0;
+    b->graph_shape = -1;
+}
+
+// Uploads the instance table and tile offsets for the current shape.  Every
+// instance must be at the same ping-pong parity; returns that parity in *parity.
+int prepare(aero_batch* b, int* parity) {
+    const int p0 = b->inst[0]->cur;
+    std::vector<StepArgs> tab(b->inst.size());
+    std::vector<uint32_t> begin(b->inst.size() + 1);
+    uint32_t acc = 0;
+    for (size_t i = 0; i < b->inst.size(); i++) {
+        aero_ctx* c = b->inst[i];
+        if (c->cur != p0) return bfail(b, AERO_E_STATE, "aero_batch: instances are at different step parities");
+        BCHK(b, hipStreamSynchronize(c->stream));
+        StepArgs a{};
+        fill_step_consts(c, &a);
+        a.fin = c->f[0];
+        a.fout = c->f[1];
+        tile_geometry(c, b->shape, &a.tiles_x, &a.tiles_y, &a.ntiles);
+        tab[i] = a;
+        begin[i] = acc;
+        acc += a.ntiles;
+    }
+    begin[b->inst.size()] = acc;
+    b->total_tiles = acc;
+    BCHK(b, hipMemcpyAsync(b->d_tab, tab.data(), tab.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipMemcpyAsync(b->d_begin, begin.data(), begin.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    BCHK(b, hipStreamSynchronize(b->stream));  // the host tables go out of scope here
+    *parity = p0;
+    return AERO_OK;
+}
+
+BatchArgs args(aero_batch* b, int parity, int write_fields) {
+    BatchArgs a{};
+    a.tab = b->d_tab;
+    a.begin = b->d_begin;
+    a.ninst = (uint32_t)b->inst.size();
+    a.parity = parity;
+    a.write_fields = write_fields;
+    return a;
+}
+
+void advance(aero_batch* b, uint32_t nsteps) {
+    for (aero_ctx* c : b->inst) {
+        if (nsteps & 1u) c->cur ^= 1;
+        c->steps += nsteps;
+    }
+}
+
+int launch_eager(aero_batch* b, int parity, uint32_t nsteps, bool fields_last) {
+    for (uint32_t k = 0; k < nsteps; k++) {
+        const int wf = (fields_last && k + 1 == nsteps) ? 1 : 0;
+        BCHK(b, launch_batch(args(b, parity ^ (int)(k & 1u), wf), b->storage, b->exact, b->shape, b->total_tiles,
+                             b->stream));
+    }
+    return AERO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aero_batch_create(uint32_t n, const aero_grid* grids, const aero_sim* sims, int storage, aero_batch** out) {
+    if (!out || !grids || !sims || n == 0 || n > 4096) return AERO_E_INVALID;
+    *out = nullptr;
+    aero_batch* b = new (std::nothrow) aero_batch();
+    if (!b) return AERO_E_NOMEM;
+    b->storage = storage & ~AERO_MATH_EXACT;
+    b->exact = (storage & AERO_MATH_EXACT) ? 1 : 0;
+    for (uint32_t i = 0; i < n; i++) {
+        aero_ctx* c = nullptr;
+        const int rc = aero_create(&grids[i], &sims[i], storage, &c);
+        if (rc != AERO_OK) {
+            for (aero_ctx* x : b->inst) aero_destroy(x);
+            delete b;
+            return rc;  // message in aero_global_error()
+        }
+        b->inst.push_back(c);
+    }
+    b->device = b->inst[0]->device;
+    uint64_t cells = 0;
+    for (aero_ctx* c : b->inst) cells += c->grid.n;
+    b->shape = cells > 8000000ull ? 6 : 0;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&b->ev1);
+    if (e == hipSuccess) e = hipMalloc(&b->d_tab, n * sizeof(StepArgs));
+    if (e == hipSuccess) e = hipMalloc(&b->d_begin, (n + 1) * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        for (aero_ctx* x : b->inst) aero_destroy(x);
+        if (b->d_tab) (void)hipFree(b->d_tab);
+        if (b->d_begin) (void)hipFree(b->d_begin);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return e == hipErrorOutOfMemory ? AERO_E_NOMEM : AERO_E_HIP;
+    }
+    *out = b;
+    return AERO_OK;
+}
+
+int aero_batch_destroy(aero_batch* b) {
+    if (!b) return AERO_E_INVALID;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    drop_graph(b);
+    for (aero_ctx* c : b->inst) aero_destroy(c);
+    if (b->d_tab) (void)hipFree(b->d_tab);
+    if (b->d_begin) (void)hipFree(b->d_begin);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return AERO_OK;
+}
+
+const char* aero_batch_last_error(const aero_batch* b) { return b ? b->err.c_str() : "null batch"; }
+
+uint32_t aero_batch_size(const aero_batch* b) { return b ? (uint32_t)b->inst.size() : 0u; }
+
+aero_ctx* aero_batch_instance(aero_batch* b, uint32_t i) {
+    if (!b || i >= b->inst.size()) return nullptr;
+    return b->inst[i];
+}
+
+int aero_batch_set_launch_shape(aero_batch* b, int shape) {
+    if (!b) return AERO_E_INVALID;
+    if (shape < 0 || shape >= kNumShapes || !batch_shape_ok(shape))
+        return bfail(b, AERO_E_INVALID, "aero_batch_set_launch_shape: shape must be one of 0,1,4,5,6,7");
+    if (shape != b->shape) drop_graph(b);
+    b->shape = shape;
+    return AERO_OK;
+}
+
+int aero_batch_get_launch_shape(aero_batch* b, int* shape) {
+    if (!b || !shape) return AERO_E_INVALID;
+    *shape = b->shape;
+    return AERO_OK;
+}
+
+int aero_batch_step(aero_batch* b, uint32_t nsteps) {
+    if (!b) return AERO_E_INVALID;
+    BCHK(b, hipSetDevice(b->device));
+    int parity = 0;
+    int rc = prepare(b, &parity);
+    if (rc != AERO_OK) return rc;
+    rc = launch_eager(b, parity, nsteps, true);
+    if (rc != AERO_OK) return rc;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    advance(b, nsteps);
+    return AERO_OK;
+}
+
+int aero_batch_run_graph(aero_batch* b, uint32_t nsteps, uint32_t steps_per_graph, float* ms) {
+    if (!b) return AERO_E_INVALID;
+    if (steps_per_graph < 2 || steps_per_graph > 4096 || (steps_per_graph & 1u))
+        return bfail(b, AERO_E_INVALID, "aero_batch_run_graph: steps_per_graph must be even, 2..4096");
+    BCHK(b, hipSetDevice(b->device));
+    int parity = 0;
+    int rc = prepare(b, &parity);
+    if (rc != AERO_OK) return rc;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    // The graph's parity arguments are fixed at capture; a graph always holds an even
+    // number of steps, so it is captured for the current parity and recaptured if the
+    // parity or the shape changed.
+    const bool reuse = b->gexec && b->graph_steps == steps_per_graph && b->graph_shape == (b->shape * 2 + parity);
+    if (!reuse) {
+        drop_graph(b);
+        hipGraph_t g = nullptr;
+        BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        rc = launch_eager(b, parity, steps_per_graph, true);
+        hipError_t e = hipStreamEndCapture(b->stream, &g);
+        if (rc != AERO_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return bhip(b, e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return bhip(b, e, "hipGraphInstantiate");
+        b->graph_steps = steps_per_graph;
+        b->graph_shape = b->shape * 2 + parity;
+    }
+    const uint32_t reps = nsteps / steps_per_graph, rest = nsteps % steps_per_graph;
+    BCHK(b, hipEventRecord(b->ev0, b->stream));
+    for (uint32_t r = 0; r < reps; r++) BCHK(b, hipGraphLaunch(b->gexec, b->stream));
+    rc = launch_eager(b, parity, rest, true);
+    if (rc != AERO_OK) return rc;
+    BCHK(b, hipEventRecord(b->ev1, b->stream));
+    BCHK(b, hipEventSynchronize(b->ev1));
+    float t = 0.0f;
+    BCHK(b, hipEventElapsedTime(&t, b->ev0, b->ev1));
+    if (ms) *ms = t;
+    advance(b, nsteps);
+    return AERO_OK;
+}
+
+int aero_batch_autotune(aero_batch* b, uint32_t steps_per_shape, int* shape_out, float* us_per_step) {
+    if (!b) return AERO_E_INVALID;
+    if (steps_per_shape == 0 || steps_per_shape > 10000)
+        return bfail(b, AERO_E_INVALID, "aero_batch_autotune: steps_per_shape must be 1..10000");
+    BCHK(b, hipSetDevice(b->device));
+    // Save every instance's newest populations, outlet plane and force slots.
+    struct Saved { void* f; void* s; void* p; };
+    std::vector<Saved> saved(b->inst.size(), Saved{nullptr, nullptr, nullptr});
+    auto release = [&]() {
+        for (auto& sv : saved) {
+            if (sv.f) (void)hipFree(sv.f);
+            if (sv.s) (void)hipFree(sv.s);
+            if (sv.p) (void)hipFree(sv.p);
+        }
+    };
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < b->inst.size() && e == hipSuccess; i++) {
+        aero_ctx* c = b->inst[i];
+        const size_t sb = (size_t)c->nslots * sizeof(TileSlot), pb = (size_t)c->grid.ny * c->grid.nz * 12;
+        e = hipMalloc(&saved[i].f, c->pop_bytes);
+        if (e == hipSuccess) e = hipMalloc(&saved[i].s, sb);
+        if (e == hipSuccess) e = hipMalloc(&saved[i].p, pb);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(saved[i].f, c->f[c->cur], c->pop_bytes, hipMemcpyDeviceToDevice, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(saved[i].s, c->slots, sb, hipMemcpyDeviceToDevice, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(saved[i].p, c->prev_u, pb, hipMemcpyDeviceToDevice, b->stream);
+    }
+    if (e != hipSuccess) { release(); return bhip(b, e, "aero_batch_autotune: save"); }
+    const int cands[6] = {0, 1, 4, 5, 6, 7};
+    const int shape0 = b->shape;
+    int best = shape0;
+    float best_us = -1.0f;
+    std::vector<int> cur0(b->inst.size());
+    std::vector<uint64_t> steps0(b->inst.size());
+    for (size_t i = 0; i < b->inst.size(); i++) {
+        cur0[i] = b->inst[i]->cur;
+        steps0[i] = b->inst[i]->steps;
+    }
+    int rc = AERO_OK;
+    for (int ci = 0; ci < 6 && rc == AERO_OK; ci++) {
+        b->shape = cands[ci];
+        int parity = 0;
+        rc = prepare(b, &parity);
+        if (rc != AERO_OK) break;
+        rc = launch_eager(b, parity, 2, false);  // warm-up
+        if (rc != AERO_OK) break;
+        if ((e = hipEventRecord(b->ev0, b->stream)) != hipSuccess) { rc = bhip(b, e, "event"); break; }
+        rc = launch_eager(b, parity, steps_per_shape, false);
+        if (rc != AERO_OK) break;
+        float ms = 0.0f;
+        if ((e = hipEventRecord(b->ev1, b->stream)) != hipSuccess || (e = hipEventSynchronize(b->ev1)) != hipSuccess ||
+            (e = hipEventElapsedTime(&ms, b->ev0, b->ev1)) != hipSuccess) {
+            rc = bhip(b, e, "event");
+            break;
+        }
+        advance(b, 2 + steps_per_shape);
+        const float us = ms * 1000.0f / (float)steps_per_shape;
+        if (best_us < 0.0f || us < best_us) { best = cands[ci]; best_us = us; }
+    }
+    // Restore the state whatever happened above.
+    for (size_t i = 0; i < b->inst.size(); i++) {
+        aero_ctx* c = b->inst[i];
+        c->cur = cur0[i];
+        const size_t sb = (size_t)c->nslots * sizeof(TileSlot), pb = (size_t)c->grid.ny * c->grid.nz * 12;
+        if (e == hipSuccess) e = hipMemcpyAsync(c->f[c->cur], saved[i].f, c->pop_bytes, hipMemcpyDeviceToDevice, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->slots, saved[i].s, sb, hipMemcpyDeviceToDevice, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->prev_u, saved[i].p, pb, hipMemcpyDeviceToDevice, b->stream);
+        c->steps = steps0[i];
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    release();
+    if (rc != AERO_OK) { b->shape = shape0; return rc; }
+    if (e != hipSuccess) return bhip(b, e, "aero_batch_autotune: restore");
+    drop_graph(b);  // a cached graph may hold the parity of the timing runs
+    b->shape = best;
+    if (shape_out) *shape_out = best;
+    if (us_per_step) *us_per_step = best_us;
+    return AERO_OK;
+}
+
+}  // extern "C"

@@ -179,14 +179,15 @@ __device__ __forceinline__ float feq_dev(float ux, float uy, float uz, float uu1
 //                  throughout, which also keeps the small departures from
 //                  equilibrium out of the cancellation against w.
 template <int S, bool EXACT, int TX>
-__device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l) {
+__device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, const void* a_fin, void* a_fout,
+                                          const int write_fields) {
     constexpr int TY = kBlock / TX;
     using ST = Store<S>;
     using T = typename ST::T;
     constexpr bool DEV = !EXACT && (S != AERO_STORE_F32);  // pulled values are f - w
     constexpr uint32_t ESZ = (uint32_t)sizeof(T);
-    const rsrc_t rin = make_rsrc(a.fin, a.pop_bytes);
-    const rsrc_t rout = make_rsrc(a.fout, a.pop_bytes);
+    const rsrc_t rin = make_rsrc(a_fin, a.pop_bytes);
+    const rsrc_t rout = make_rsrc(a_fout, a.pop_bytes);
     const uint32_t pbytes = a.plane_bytes;  // bytes per population plane
 
     const uint32_t txy = a.tiles_x * a.tiles_y;
@@ -235,7 +236,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l) {
                 constexpr int I = decltype(ic)::value;
                 SV(idx * ESZ, (uint32_t)I * pbytes, DEV ? 0.0f : wt(I), wt(I));
             });
-            if (a.write_fields) a.fields[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (write_fields) a.fields[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (outlet) { a.prev_u[pidx] = 0.f; a.prev_u[pidx + 1] = 0.f; a.prev_u[pidx + 2] = 0.f; }
         } else {
             const bool near = (fl & FL_NEAR) != 0;
@@ -564,7 +565,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l) {
                     SV(idx * ESZ, (uint32_t)I * pbytes, o[I], wt(I));
                 });
             }
-            if (a.write_fields) a.fields[idx] = make_float4(ux, uy, uz, rho);
+            if (write_fields) a.fields[idx] = make_float4(ux, uy, uz, rho);
             if (last_tile && outlet) { a.prev_u[pidx] = ux; a.prev_u[pidx + 1] = uy; a.prev_u[pidx + 2] = uz; }
         }
     }
@@ -606,7 +607,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l) {
 template <int S, bool EXACT, int TX, bool SWZ>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     static_assert(TX >= 64 && TX <= kBlock && kBlock % TX == 0, "whole wavefronts per row");
-    step_tile<S, EXACT, TX>(a, SWZ ? xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x);
+    step_tile<S, EXACT, TX>(a, SWZ ? xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x, a.fin, a.fout, a.write_fields);
 }
 // Persistent form: kPersistBlocksPerCU resident blocks per CU walk the tiles; the
 // register budget is capped so that many blocks really are resident.
@@ -616,7 +617,25 @@ k_step_persist(StepArgs a) {
     static_assert(TX >= 64 && TX <= kBlock && kBlock % TX == 0, "whole wavefronts per row");
     // gridDim.x is a multiple of 8, so block b stays on XCD b % 8 for every tile.
     for (uint32_t b = blockIdx.x; b < a.ntiles; b += gridDim.x)
-        step_tile<S, EXACT, TX>(a, SWZ ? xcd_swizzle(b, a.ntiles) : b);
+        step_tile<S, EXACT, TX>(a, SWZ ? xcd_swizzle(b, a.ntiles) : b, a.fin, a.fout, a.write_fields);
+}
+
+// Batched form: one launch steps every tile of every tunnel of a batch.  Tiles are
+// numbered instance after instance; a block finds its instance by binary search
+// in the (wave-uniform) tile offsets and runs the same tile body with that
+// instance's constants.  All instances share the ping-pong parity.
+template <int S, bool EXACT, int TX, bool SWZ>
+__global__ void __launch_bounds__(kBlock) k_step_batch(BatchArgs b) {
+    const uint32_t g = SWZ ? xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
+    uint32_t lo = 0, hi = b.ninst;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (b.begin[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    const StepArgs& a = b.tab[lo];
+    step_tile<S, EXACT, TX>(a, g - b.begin[lo], b.parity ? a.fout : a.fin, b.parity ? (void*)a.fin : a.fout,
+                            b.write_fields);
 }
 
 // LBM_INIT (wgsl.ts:147-154) on both buffers: feq(i, 1, (uIn,0,0)) everywhere,
@@ -853,6 +872,38 @@ hipError_t launch_step_shape(aero_ctx* c, int write_fields, int shape) {
 }
 
 hipError_t launch_step(aero_ctx* c, int write_fields) { return launch_step_shape(c, write_fields, c->shape); }
+
+namespace {
+template <int S, bool EXACT>
+hipError_t launch_batch_t(const BatchArgs& b, int shape, uint32_t grid, hipStream_t st) {
+#define AERO_B(TX, SWZ) hipLaunchKernelGGL((k_step_batch<S, EXACT, TX, SWZ>), dim3(grid), dim3(kBlock), 0, st, b)
+    switch (shape) {
+    case 0: AERO_B(64, false); break;
+    case 1: AERO_B(64, true); break;
+    case 4: AERO_B(128, false); break;
+    case 5: AERO_B(128, true); break;
+    case 6: AERO_B(256, false); break;
+    case 7: AERO_B(256, true); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef AERO_B
+    return hipGetLastError();
+}
+}  // namespace
+
+bool batch_shape_ok(int shape) { return shape == 0 || shape == 1 || shape == 4 || shape == 5 || shape == 6 || shape == 7; }
+
+hipError_t launch_batch(const BatchArgs& b, int storage, int exact, int shape, uint32_t total_tiles, hipStream_t st) {
+    if (!batch_shape_ok(shape)) return hipErrorInvalidValue;
+    switch (storage) {
+    case AERO_STORE_F32: return exact ? launch_batch_t<AERO_STORE_F32, true>(b, shape, total_tiles, st)
+                                      : launch_batch_t<AERO_STORE_F32, false>(b, shape, total_tiles, st);
+    case AERO_STORE_F16S: return exact ? launch_batch_t<AERO_STORE_F16S, true>(b, shape, total_tiles, st)
+                                       : launch_batch_t<AERO_STORE_F16S, false>(b, shape, total_tiles, st);
+    default: return exact ? launch_batch_t<AERO_STORE_BF16S, true>(b, shape, total_tiles, st)
+                          : launch_batch_t<AERO_STORE_BF16S, false>(b, shape, total_tiles, st);
+    }
+}
 
 hipError_t launch_reduce_slots(aero_ctx* c) {
     TileSlot* parts = (TileSlot*)c->red_out;

@@ -183,6 +183,35 @@ int aero_write_populations(aero_ctx* ctx, const float* in_19n); /* newest buffer
 int aero_guard_count(aero_ctx* ctx, uint64_t* count);
 int aero_sync(aero_ctx* ctx);
 
+/* ---- Batched tunnels ----
+ * Replaces the reference's sequential ride-height sweep (src/ui/aeromap.ts:73-114,
+ * one setHeights → settle → measure period per map point on the one engine) with
+ * n independent tunnels of any sizes, stepped together by one fused launch per
+ * step and replayed from a captured HIP graph.  Each instance is an ordinary
+ * context: voxelize, init, set_sim, read_forces etc. work on it unchanged
+ * (aero_batch_instance); it must not be stepped on its own while it belongs to a
+ * batch, and aero_destroy must not be called on it (aero_batch_destroy does).
+ * All instances share one storage mode (storage argument as for aero_create). */
+typedef struct aero_batch aero_batch;
+int aero_batch_create(uint32_t n, const aero_grid* grids, const aero_sim* sims, int storage, aero_batch** out);
+int aero_batch_destroy(aero_batch* batch);
+const char* aero_batch_last_error(const aero_batch* batch);
+uint32_t aero_batch_size(const aero_batch* batch);
+aero_ctx* aero_batch_instance(aero_batch* batch, uint32_t i);
+/* nsteps fused steps of every instance, launched one by one (fields and the outlet
+ * plane are written on the last step, as aero_step does). */
+int aero_batch_step(aero_batch* batch, uint32_t nsteps);
+/* nsteps fused steps: nsteps / steps_per_graph replays of a captured graph of
+ * steps_per_graph (even, 2..4096) steps, then the remainder launched one by one.
+ * The graph is cached and recaptured only when the launch shape or the instances'
+ * step parity changes.  *ms (nullable) receives the device time of the whole run. */
+int aero_batch_run_graph(aero_batch* batch, uint32_t nsteps, uint32_t steps_per_graph, float* ms);
+/* Times every batch launch shape (0,1,4,5,6,7) for steps_per_shape steps on a saved
+ * copy of every instance's state, restores the state and keeps the fastest. */
+int aero_batch_autotune(aero_batch* batch, uint32_t steps_per_shape, int* shape, float* us_per_step);
+int aero_batch_set_launch_shape(aero_batch* batch, int shape);
+int aero_batch_get_launch_shape(aero_batch* batch, int* shape);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,7 +79,7 @@ def as_prims(prims):
 
 
 class LBMOracle:
-    """f32 ('f32'), f64 ('f64') or OpenMP f32 ('f32_omp') restatement of LBM_INIT /
+    """f32 ('f32'), f64 ('f64') or OpenMP ('f32_omp', 'f64_omp') restatement of LBM_INIT /
     LBM_STEP / HEAT."""
 
     def __init__(self, kind: str = "f32"):

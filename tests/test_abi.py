@@ -79,3 +79,20 @@ def test_shape_uniform_packing_matches_engine():
     assert s.rear_flap == pytest.approx((-3 + 32) * math.pi / 180, rel=1e-7)
     assert s.heat_brakes == 520 and s.heat_engine == 0 and s.heat_rad == pytest.approx(75 * 0.6)
     assert s.duct == pytest.approx(1.3) and s.pw == 256 and s.ph == 128 and s.p_scale == pytest.approx(0.02)
+
+
+def test_batch_entry_points_reject_bad_arguments_before_any_gpu_call():
+    lib = _lib.load()
+    b = C.c_void_p()
+    g = (_lib.Grid * 1)(_lib.Grid(16, 8, 8, 1024, 0, 0, 0, 0.1))
+    s = (_lib.Sim * 1)(_lib.Sim(0.05, 0.51, 0.1, 0.0))
+    assert lib.aero_batch_create(0, g, s, 0, C.byref(b)) == -1
+    assert lib.aero_batch_create(1, None, s, 0, C.byref(b)) == -1
+    assert lib.aero_batch_create(1, g, s, 0, None) == -1
+    assert lib.aero_batch_step(None, 1) == -1
+    assert lib.aero_batch_run_graph(None, 2, 2, None) == -1
+    assert lib.aero_batch_autotune(None, 1, None, None) == -1
+    assert lib.aero_batch_destroy(None) == -1
+    assert lib.aero_batch_instance(None, 0) is None
+    assert lib.aero_batch_size(None) == 0
+    assert lib.aero_batch_last_error(None) == b"null batch"
