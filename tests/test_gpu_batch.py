@@ -220,3 +220,32 @@ def test_aeromap_with_uneven_tunnels_matches_separate_batches():
     for p, q in zip(both.points[:25], alone.points):
         assert np.array_equal(p.sums, q.sums)
     assert all(np.isfinite(p.fx) and p.fx > 0 for p in both.points[25:])
+
+
+def test_aeromap_window_forces_are_rounding_limited_against_fp64():
+    """One map point (front 20 mm, rear 40 mm), 2000 steps, forces averaged over steps
+    1001-2000, against the FP64 oracle: the reference's own FP32 arithmetic (exact
+    mode), FP32 with a 1e-7 relative perturbation of the initial state, and FP16S.
+    All three land within a few percent of FP64, and FP16S no further than the FP32
+    runs spread (the flow's shear layers amplify rounding after about step 500)."""
+    w = W.aeromap(2000, uneven=False)[0]
+    ref = fp64_point(w, 1000, 1000)
+    res = {}
+    for name, storage, math, pert in (("f32 exact", "f32", "exact", 0.0), ("f32 exact + 1e-7", "f32", "exact", 1e-7),
+                                      ("f16s", "f16s", "fast", 0.0)):
+        e = aerolbm.HipAeroEngine(storage=storage, math=math, **w.engine_kwargs())
+        e.set_shape(w.shape, prims=w.prims)
+        e.reset()
+        if pert:
+            p = e.populations()
+            rng = np.random.default_rng(1)
+            e.write_populations((p * (1 + pert * rng.standard_normal(p.shape))).astype(np.float32))
+        e.step(1000)
+        e.read_forces_raw(reset=True)
+        e.step(1000)
+        d, _, n = e.read_forces_raw(reset=True)
+        res[name] = rel_force_err(aeromap._point(w, d, n), ref)
+        e.close()
+    print("  window error vs FP64:", {k: f"{v:.2e}" for k, v in res.items()}, flush=True)
+    assert max(res.values()) <= 0.10
+    assert res["f16s"] <= 2.0 * max(res["f32 exact"], res["f32 exact + 1e-7"]) + 0.005
