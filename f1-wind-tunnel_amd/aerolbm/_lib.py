@@ -54,6 +54,12 @@ class Prim(C.Structure):
                 ("a", C.c_float * 12)]
 
 
+class AdamW(C.Structure):
+    """aero_adamw."""
+    _fields_ = [("lr", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+                ("weight_decay", C.c_float), ("pad", C.c_uint32 * 3)]
+
+
 class Rig(C.Structure):
     """aero_rig == WGSL struct Rig (src/engine/car-model.ts:60-69)."""
     _fields_ = [("front", (C.c_float * 4) * 71), ("hinges", C.c_float * 4), ("angles", C.c_float * 4),
@@ -104,6 +110,17 @@ _PROTOS = {
     "aero_batch_autotune": (C.c_int, [_P, _U32, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "aero_batch_set_launch_shape": (C.c_int, [_P, C.c_int]),
     "aero_batch_get_launch_shape": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "aero_mlp_create": (C.c_int, [_U32, _U32, _P, _U32, C.POINTER(_P)]),
+    "aero_mlp_destroy": (C.c_int, [_P]),
+    "aero_mlp_last_error": (C.c_char_p, [_P]),
+    "aero_mlp_param_count": (C.c_uint64, [_P]),
+    "aero_mlp_set_params": (C.c_int, [_P, _P]),
+    "aero_mlp_get_params": (C.c_int, [_P, _P]),
+    "aero_mlp_reset_optimizer": (C.c_int, [_P]),
+    "aero_mlp_forward": (C.c_int, [_P, _P, _U32, _P, C.POINTER(C.c_float)]),
+    "aero_mlp_loss_grad": (C.c_int, [_P, _P, _P, _U32, _P, _P]),
+    "aero_mlp_set_data": (C.c_int, [_P, _P, _P, _U32]),
+    "aero_mlp_train": (C.c_int, [_P, _P, _U32, _U32, _P, _P, C.POINTER(C.c_float)]),
 }
 
 _lib = None
@@ -132,11 +149,13 @@ def exported_symbols() -> list[str]:
     return list(_PROTOS)
 
 
-def check(rc: int, ctx=None, what: str = "", batch=None) -> None:
+def check(rc: int, ctx=None, what: str = "", batch=None, mlp=None) -> None:
     if rc == AERO_OK:
         return
     lib = load()
-    if batch:
+    if mlp:
+        msg = lib.aero_mlp_last_error(mlp) or b""
+    elif batch:
         msg = lib.aero_batch_last_error(batch) or b""
     else:
         msg = (lib.aero_last_error(ctx) if ctx else lib.aero_global_error()) or b""

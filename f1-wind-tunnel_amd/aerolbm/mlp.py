@@ -1,0 +1,122 @@
+"""Ensemble of MLPs trained and evaluated by the library's BF16-MFMA kernels
+(aero_mlp_* in include/aero_lbm.h).  The network, its loss and its optimizer live on
+the GPU; this module only packs arguments, draws bootstrap indices and initialises
+weights."""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import AdamW, check, ptr
+
+SURROGATE_DIMS = (16, 256, 256, 256, 256, 4)
+
+
+class MLPEnsemble:
+    def __init__(self, dims: Sequence[int] = SURROGATE_DIMS, members: int = 5, max_batch: int = 4096,
+                 seed: int = 0):
+        self._lib = _lib.load()
+        self.dims = tuple(int(d) for d in dims)
+        self.members = int(members)
+        self.max_batch = int(max_batch)
+        d = np.array(self.dims, np.uint32)
+        h = C.c_void_p()
+        check(self._lib.aero_mlp_create(self.members, len(self.dims) - 1, ptr(d), self.max_batch, C.byref(h)), None,
+              "aero_mlp_create")
+        self.h = h
+        self.per = int(self._lib.aero_mlp_param_count(h))
+        self.rng = np.random.default_rng(seed)
+        self.ndata = 0
+        self.set_params(self.init_params())
+
+    def _c(self, rc: int, what: str) -> None:
+        check(rc, None, what, mlp=self.h)
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self._lib.aero_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_params(self) -> np.ndarray:
+        """He-normal weights (std sqrt(2 / fan_in)), zero biases; the output layer is
+        scaled down by 10 so that an untrained member predicts about zero."""
+        out = np.empty((self.members, self.per), np.float32)
+        L = len(self.dims) - 1
+        for m in range(self.members):
+            parts = []
+            for l in range(L):
+                n, k = self.dims[l + 1], self.dims[l]
+                W = self.rng.standard_normal((n, k)) * np.sqrt(2.0 / k)
+                if l == L - 1:
+                    W *= 0.1
+                parts += [W.reshape(-1), np.zeros(n)]
+            out[m] = np.concatenate(parts)
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        a = np.ascontiguousarray(p, np.float32).reshape(self.members, self.per)
+        self._c(self._lib.aero_mlp_set_params(self.h, ptr(a)), "aero_mlp_set_params")
+
+    def params(self) -> np.ndarray:
+        out = np.empty((self.members, self.per), np.float32)
+        self._c(self._lib.aero_mlp_get_params(self.h, ptr(out)), "aero_mlp_get_params")
+        return out
+
+    def forward(self, x: np.ndarray) -> Tuple[np.ndarray, float]:
+        """x [B, d0] -> (y [members, B, dL], device ms)."""
+        xa = np.ascontiguousarray(x, np.float32)
+        B = xa.shape[0]
+        y = np.empty((self.members, B, self.dims[-1]), np.float32)
+        ms = C.c_float()
+        self._c(self._lib.aero_mlp_forward(self.h, ptr(xa), B, ptr(y), C.byref(ms)), "aero_mlp_forward")
+        return y, float(ms.value)
+
+    def loss_grad(self, x: np.ndarray, t: np.ndarray):
+        """x [members, B, d0], t [members, B, dL] -> (loss [members], grads [members, P])."""
+        xa = np.ascontiguousarray(x, np.float32)
+        ta = np.ascontiguousarray(t, np.float32)
+        B = xa.shape[1]
+        loss = np.empty(self.members, np.float32)
+        g = np.empty((self.members, self.per), np.float32)
+        self._c(self._lib.aero_mlp_loss_grad(self.h, ptr(xa), ptr(ta), B, ptr(loss), ptr(g)), "aero_mlp_loss_grad")
+        return loss, g
+
+    def set_data(self, X: np.ndarray, T: np.ndarray) -> None:
+        xa = np.ascontiguousarray(X, np.float32)
+        ta = np.ascontiguousarray(T, np.float32)
+        self.ndata = xa.shape[0]
+        self._c(self._lib.aero_mlp_set_data(self.h, ptr(xa), ptr(ta), self.ndata), "aero_mlp_set_data")
+
+    def reset_optimizer(self) -> None:
+        self._c(self._lib.aero_mlp_reset_optimizer(self.h), "aero_mlp_reset_optimizer")
+
+    def bootstrap_indices(self, nsteps: int, batch: int, rows: Optional[np.ndarray] = None) -> np.ndarray:
+        """Each member draws its minibatches from its own bootstrap resample of the rows."""
+        rows = np.arange(self.ndata) if rows is None else np.asarray(rows)
+        idx = np.empty((nsteps, self.members, batch), np.uint32)
+        for m in range(self.members):
+            boot = self.rng.choice(rows, size=rows.size, replace=True)
+            idx[:, m, :] = self.rng.choice(boot, size=(nsteps, batch), replace=True)
+        return idx
+
+    def train(self, nsteps: int, batch: int = 256, lr: float = 1e-3, weight_decay: float = 1e-4,
+              idx: Optional[np.ndarray] = None, betas=(0.9, 0.999), eps: float = 1e-8):
+        """nsteps AdamW steps; returns (loss per step, device ms)."""
+        if idx is None:
+            idx = self.bootstrap_indices(nsteps, batch)
+        ia = np.ascontiguousarray(idx, np.uint32)
+        opt = AdamW(lr, betas[0], betas[1], eps, weight_decay)
+        loss = np.empty(nsteps, np.float32)
+        ms = C.c_float()
+        self._c(self._lib.aero_mlp_train(self.h, ptr(ia), nsteps, batch, C.byref(opt), ptr(loss), C.byref(ms)),
+                "aero_mlp_train")
+        return loss, float(ms.value)

@@ -1,0 +1,670 @@
+// MLP surrogate (aero_mlp_*): an ensemble of small fully connected networks mapping a
+// car setup to its force coefficients, trained and evaluated on the GPU.
+//
+// Every member has the same widths d0 -> d1 -> ... -> dL with GELU after each hidden
+// layer.  Layer inputs, weights and back-propagated errors are BF16 operands of
+// v_mfma_f32_16x16x16bf16_1k with FP32 accumulation; master weights, biases,
+// pre-activations, gradients and the AdamW moments are FP32.  All members run in the
+// same launches (grid z = member).
+//
+// There is no counterpart in the reference, whose setup-to-force answer always comes
+// from the LBM itself (src/ui/aeromap.ts:73-114 sweeps by running it).
+#include <math.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "aero_internal.h"
+
+namespace {
+
+using bf16x4 = __attribute__((ext_vector_type(4))) short;
+using f32x4 = __attribute__((ext_vector_type(4))) float;
+
+constexpr int kTM = 64, kTN = 64, kKC = 32, kLdsK = kKC + 8;  // LDS rows padded to 80 B
+
+// FP32 -> BF16, round to nearest even (NaN kept quiet).
+__host__ __device__ inline uint16_t to_bf16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float from_bf16(uint16_t h) {
+    const uint32_t u = (uint32_t)h << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// GELU(x) = x Phi(x) and its derivative Phi(x) + x phi(x).
+__device__ inline float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+__device__ inline float gelu_grad(float x) {
+    return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+}
+
+enum Epi : int {
+    EPI_BIAS = 0,       // outf = acc + bias
+    EPI_BIAS_GELU = 1,  // outf = acc + bias (pre-activation), outh = bf16(gelu(outf))
+    EPI_STORE = 2,      // outf = acc
+    EPI_DGELU = 3,      // outh = bf16(acc * gelu'(aux))
+};
+
+// C[e] (M x N) = A[e] (M x K) * B[e] (K x N); A and B are BF16 views given by element
+// strides, so transposed operands need no copy.  Out-of-range rows/columns/k read 0.
+struct GemmArgs {
+    const uint16_t* A;
+    long long a_e, a_m, a_k;
+    const uint16_t* B;
+    long long b_e, b_k, b_n;
+    int M, N, K;
+    const float* bias;
+    long long bias_e;
+    float* outf;
+    long long of_e, of_m;
+    uint16_t* outh;
+    long long oh_e, oh_m;
+    const float* aux;
+    long long aux_e, aux_m;
+};
+
+// 256 threads, a 64x64 tile of C per block, four waves of 32x32 (2x2 MFMA tiles),
+// K staged through LDS 32 at a time with both operands stored k-contiguous.
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[kTM][kLdsK];
+    __shared__ __attribute__((aligned(16))) uint16_t Bt[kTN][kLdsK];
+    const int e = blockIdx.z;
+    const int m0 = blockIdx.x * kTM, n0 = blockIdx.y * kTN;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+    const uint16_t* A = g.A + e * g.a_e;
+    const uint16_t* B = g.B + e * g.b_e;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // staging coordinates: A row t/4, k (t%4)*8..+7; B column t/4, k (t%4)*8..+7
+    const int sr = t >> 2, sk = (t & 3) * 8;
+    for (int k0 = 0; k0 < g.K; k0 += kKC) {
+        const int m = m0 + sr, n = n0 + sr;
+#pragma unroll
+        for (int v = 0; v < 8; v++) {
+            const int k = k0 + sk + v;
+            As[sr][sk + v] = (m < g.M && k < g.K) ? A[m * g.a_m + k * g.a_k] : (uint16_t)0;
+            Bt[sr][sk + v] = (n < g.N && k < g.K) ? B[k * g.b_k + n * g.b_n] : (uint16_t)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kKC; ks += 16) {
+            const int kk = ks + 4 * (lane >> 4);
+            bf16x4 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) a[i] = *(const bf16x4*)&As[wr + i * 16 + (lane & 15)][kk];
+#pragma unroll
+            for (int j = 0; j < 2; j++) b[j] = *(const bf16x4*)&Bt[wc + j * 16 + (lane & 15)][kk];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D layout of the 16x16 MFMA: lane l, element r -> row 4*(l/16) + r, column l % 16.
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int m = m0 + wr + i * 16 + 4 * (lane >> 4) + r;
+                const int n = n0 + wc + j * 16 + (lane & 15);
+                if (m >= g.M || n >= g.N) continue;
+                const float v = acc[i][j][r];
+                if constexpr (EPI == EPI_BIAS || EPI == EPI_BIAS_GELU) {
+                    const float pre = v + g.bias[e * g.bias_e + n];
+                    g.outf[e * g.of_e + m * g.of_m + n] = pre;
+                    if constexpr (EPI == EPI_BIAS_GELU) g.outh[e * g.oh_e + m * g.oh_m + n] = to_bf16(gelu(pre));
+                } else if constexpr (EPI == EPI_STORE) {
+                    g.outf[e * g.of_e + m * g.of_m + n] = v;
+                } else {
+                    const float z = g.aux[e * g.aux_e + m * g.aux_m + n];
+                    g.outh[e * g.oh_e + m * g.oh_m + n] = to_bf16(v * gelu_grad(z));
+                }
+            }
+}
+
+// Training inputs of one step: member e's samples idx[e*batch + i] of the device data.
+__global__ void k_gather(const float* X, const float* T, const uint32_t* idx, const uint32_t* step,
+                         uint32_t idx_stride, uint32_t batch, uint32_t d0, uint32_t dl, uint16_t* act0, float* tmb) {
+    const uint32_t e = blockIdx.y;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= batch) return;
+    const uint32_t s = idx[(size_t)(*step) * idx_stride + (size_t)e * batch + i];
+    for (uint32_t k = 0; k < d0; k++) act0[((size_t)e * batch + i) * d0 + k] = to_bf16(X[(size_t)s * d0 + k]);
+    for (uint32_t k = 0; k < dl; k++) tmb[((size_t)e * batch + i) * dl + k] = T[(size_t)s * dl + k];
+}
+
+__global__ void k_to_bf16(const float* x, uint16_t* y, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = to_bf16(x[i]);
+}
+
+// MSE over batch x dl per member; dZ = 2 (y - t) / (batch dl) in BF16; loss (mean over
+// members) into loss_buf[*step] when loss_buf is set; *step advances afterwards.
+__global__ void __launch_bounds__(256) k_loss(const float* y, const float* t, uint32_t members, uint32_t batch,
+                                              uint32_t dl, uint16_t* dz, float* loss_buf, uint32_t loss_cap,
+                                              uint32_t* step, float* loss_members) {
+    __shared__ float red[256];
+    const uint32_t per = batch * dl;
+    const float scale = 2.0f / (float)per;
+    float total = 0.f;
+    for (uint32_t e = 0; e < members; e++) {
+        float s = 0.f;
+        for (uint32_t i = threadIdx.x; i < per; i += 256) {
+            const size_t o = (size_t)e * per + i;
+            const float d = y[o] - t[o];
+            s += d * d;
+            dz[o] = to_bf16(scale * d);
+        }
+        red[threadIdx.x] = s;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const float le = red[0] / (float)per;
+            if (loss_members) loss_members[e] = le;
+            total += le;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (loss_buf) loss_buf[*step % loss_cap] = total / (float)members;
+        *step += 1u;
+    }
+}
+
+// db[e][n] = sum_m dz[e][m][n]
+__global__ void k_colsum(const uint16_t* dz, uint32_t batch, uint32_t n, float* db, long long db_e) {
+    const uint32_t e = blockIdx.y;
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float s = 0.f;
+    for (uint32_t m = 0; m < batch; m++) s += from_bf16(dz[((size_t)e * batch + m) * n + j]);
+    db[e * db_e + j] = s;
+}
+
+struct AdamArgs {
+    float* p;
+    const float* g;
+    float* m;
+    float* v;
+    uint16_t* pb;
+    const uint8_t* decay;  // 1 for weights, 0 for biases
+    size_t per, total;
+    float lr, b1, b2, eps, wd;
+    const uint32_t* step;  // optimizer step count after this update = *step
+};
+
+__global__ void k_adamw(AdamArgs a) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.total) return;
+    const float tstep = (float)(*a.step);
+    const float g = a.g[i];
+    const float m = a.b1 * a.m[i] + (1.0f - a.b1) * g;
+    const float v = a.b2 * a.v[i] + (1.0f - a.b2) * g * g;
+    a.m[i] = m;
+    a.v[i] = v;
+    const float mh = m / (1.0f - powf(a.b1, tstep));
+    const float vh = v / (1.0f - powf(a.b2, tstep));
+    float p = a.p[i];
+    if (a.decay[i % a.per]) p -= a.lr * a.wd * p;
+    p -= a.lr * mh / (sqrtf(vh) + a.eps);
+    a.p[i] = p;
+    a.pb[i] = to_bf16(p);
+}
+
+}  // namespace
+
+struct aero_mlp {
+    uint32_t members = 0, L = 0, max_batch = 0;
+    std::vector<uint32_t> dims;
+    std::vector<size_t> w_off, b_off;  // per-layer offsets into a member's parameters
+    size_t per = 0;                     // parameters per member
+    uint32_t maxd = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    float* p = nullptr;        // [members][per] FP32 master
+    uint16_t* pb = nullptr;    // BF16 copy (weights read by the MFMA)
+    float* grad = nullptr;
+    float* am = nullptr;
+    float* av = nullptr;
+    uint8_t* decay = nullptr;
+    std::vector<uint16_t*> act;  // act[l]: [members][batch][dims[l]] BF16 layer inputs
+    std::vector<float*> pre;     // pre[l]: [members][batch][dims[l+1]] FP32 pre-activations
+    uint16_t* dz[2] = {nullptr, nullptr};
+    float* xin = nullptr;        // FP32 staging [members][batch][d0]
+    float* tgt = nullptr;        // [members][batch][dL]
+    float* lossm = nullptr;      // [members]
+    float* X = nullptr;          // training data
+    float* T = nullptr;
+    uint32_t ndata = 0;
+    uint32_t* idx = nullptr;
+    size_t idx_cap = 0;
+    uint32_t* step = nullptr;    // [0] optimizer step, [1] training step within a call
+    float* loss_buf = nullptr;
+    uint32_t loss_cap = 65536;
+    uint32_t adam_t = 0;
+    hipGraphExec_t gexec = nullptr;
+    uint32_t g_batch = 0;
+    aero_adamw g_opt{};
+    std::string err;
+};
+
+namespace {
+
+int mfail(aero_mlp* m, int code, const std::string& s) {
+    if (m) m->err = s;
+    return code;
+}
+int mhip(aero_mlp* m, hipError_t e, const char* where) {
+    return mfail(m, e == hipErrorOutOfMemory ? AERO_E_NOMEM : AERO_E_HIP,
+                 std::string(where) + ": " + hipGetErrorName(e) + " (" + hipGetErrorString(e) + ")");
+}
+#define MCHK(m, expr)                                       \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return mhip((m), e_, #expr);  \
+    } while (0)
+
+dim3 gemm_grid(int M, int N, uint32_t E) { return dim3((M + kTM - 1) / kTM, (N + kTN - 1) / kTN, E); }
+
+template <int EPI>
+hipError_t gemm(aero_mlp* m, const GemmArgs& g, uint32_t E) {
+    hipLaunchKernelGGL(k_gemm<EPI>, gemm_grid(g.M, g.N, E), dim3(256), 0, m->stream, g);
+    return hipGetLastError();
+}
+
+// Forward through every layer.  shared_input: act[0] holds one [batch][d0] block read
+// by all members (member stride 0).
+hipError_t forward(aero_mlp* m, uint32_t batch, bool shared_input) {
+    const long long per = (long long)m->per;
+    for (uint32_t l = 0; l < m->L; l++) {
+        const int K = (int)m->dims[l], N = (int)m->dims[l + 1];
+        GemmArgs g{};
+        g.A = m->act[l];
+        g.a_e = (l == 0 && shared_input) ? 0 : (long long)batch * K;
+        g.a_m = K;
+        g.a_k = 1;
+        g.B = m->pb + m->w_off[l];  // W [N][K]: B[k][n] = W[n][k]
+        g.b_e = per;
+        g.b_k = 1;
+        g.b_n = K;
+        g.M = (int)batch;
+        g.N = N;
+        g.K = K;
+        g.bias = m->p + m->b_off[l];
+        g.bias_e = per;
+        g.outf = m->pre[l];
+        g.of_e = (long long)batch * N;
+        g.of_m = N;
+        hipError_t e;
+        if (l + 1 < m->L) {
+            g.outh = m->act[l + 1];
+            g.oh_e = (long long)batch * N;
+            g.oh_m = N;
+            e = gemm<EPI_BIAS_GELU>(m, g, m->members);
+        } else {
+            e = gemm<EPI_BIAS>(m, g, m->members);
+        }
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Backward from dz[cur] (the output-layer error) into m->grad.
+hipError_t backward(aero_mlp* m, uint32_t batch) {
+    const long long per = (long long)m->per;
+    int cur = 0;
+    for (int l = (int)m->L - 1; l >= 0; l--) {
+        const int K = (int)m->dims[l], N = (int)m->dims[l + 1];
+        const uint16_t* dzl = m->dz[cur];
+        {  // dW[n][k] = sum_m dz[m][n] act[m][k]
+            GemmArgs g{};
+            g.A = dzl;
+            g.a_e = (long long)batch * N;
+            g.a_m = 1;
+            g.a_k = N;
+            g.B = m->act[l];
+            g.b_e = (long long)batch * K;
+            g.b_k = K;
+            g.b_n = 1;
+            g.M = N;
+            g.N = K;
+            g.K = (int)batch;
+            g.outf = m->grad + m->w_off[l];
+            g.of_e = per;
+            g.of_m = K;
+            hipError_t e = gemm<EPI_STORE>(m, g, m->members);
+            if (e != hipSuccess) return e;
+        }
+        hipLaunchKernelGGL(k_colsum, dim3((N + 255) / 256, m->members), dim3(256), 0, m->stream, dzl, batch,
+                           (uint32_t)N, m->grad + m->b_off[l], per);
+        if (l > 0) {  // dz_prev[m][k] = (sum_n dz[m][n] W[n][k]) * gelu'(pre[l-1][m][k])
+            GemmArgs g{};
+            g.A = dzl;
+            g.a_e = (long long)batch * N;
+            g.a_m = N;
+            g.a_k = 1;
+            g.B = m->pb + m->w_off[l];
+            g.b_e = per;
+            g.b_k = K;
+            g.b_n = 1;
+            g.M = (int)batch;
+            g.N = K;
+            g.K = N;
+            g.outh = m->dz[cur ^ 1];
+            g.oh_e = (long long)batch * K;
+            g.oh_m = K;
+            g.aux = m->pre[l - 1];
+            g.aux_e = (long long)batch * K;
+            g.aux_m = K;
+            hipError_t e = gemm<EPI_DGELU>(m, g, m->members);
+            if (e != hipSuccess) return e;
+            cur ^= 1;
+        }
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_loss(aero_mlp* m, uint32_t batch, float* loss_buf) {
+    const uint32_t dl = m->dims[m->L];
+    hipLaunchKernelGGL(k_loss, dim3(1), dim3(256), 0, m->stream, m->pre[m->L - 1], m->tgt, m->members, batch, dl,
+                       m->dz[0], loss_buf, m->loss_cap, m->step + 1, m->lossm);
+    return hipGetLastError();
+}
+
+hipError_t launch_adam(aero_mlp* m, const aero_adamw& o) {
+    AdamArgs a{};
+    a.p = m->p;
+    a.g = m->grad;
+    a.m = m->am;
+    a.v = m->av;
+    a.pb = m->pb;
+    a.decay = m->decay;
+    a.per = m->per;
+    a.total = m->per * m->members;
+    a.lr = o.lr;
+    a.b1 = o.beta1;
+    a.b2 = o.beta2;
+    a.eps = o.eps;
+    a.wd = o.weight_decay;
+    a.step = m->step;
+    hipLaunchKernelGGL(k_adamw, dim3((unsigned)((a.total + 255) / 256)), dim3(256), 0, m->stream, a);
+    return hipGetLastError();
+}
+
+__global__ void k_tick(uint32_t* optimizer_step) { *optimizer_step += 1u; }
+
+// One training step: gather, forward, loss, backward, optimizer-step count, AdamW.
+hipError_t train_step(aero_mlp* m, uint32_t batch, const aero_adamw& o) {
+    const uint32_t d0 = m->dims[0], dl = m->dims[m->L];
+    hipLaunchKernelGGL(k_gather, dim3((batch + 255) / 256, m->members), dim3(256), 0, m->stream, m->X, m->T, m->idx,
+                       m->step + 1, m->members * batch, batch, d0, dl, m->act[0], m->tgt);
+    hipError_t e = forward(m, batch, false);
+    if (e == hipSuccess) e = launch_loss(m, batch, m->loss_buf);
+    if (e == hipSuccess) e = backward(m, batch);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_tick, dim3(1), dim3(1), 0, m->stream, m->step);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = launch_adam(m, o);
+    return e;
+}
+
+void drop_graph(aero_mlp* m) {
+    if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+    m->gexec = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aero_mlp_create(uint32_t members, uint32_t nlayers, const uint32_t* dims, uint32_t max_batch, aero_mlp** out) {
+    if (!out || !dims || members == 0 || members > 64 || nlayers == 0 || nlayers > 16 || max_batch == 0 ||
+        max_batch > (1u << 20))
+        return AERO_E_INVALID;
+    *out = nullptr;
+    for (uint32_t l = 0; l <= nlayers; l++)
+        if (dims[l] == 0 || dims[l] > 4096) return AERO_E_INVALID;
+    aero_mlp* m = new (std::nothrow) aero_mlp();
+    if (!m) return AERO_E_NOMEM;
+    m->members = members;
+    m->L = nlayers;
+    m->max_batch = max_batch;
+    m->dims.assign(dims, dims + nlayers + 1);
+    for (uint32_t l = 0; l < nlayers; l++) {
+        m->w_off.push_back(m->per);
+        m->per += (size_t)dims[l + 1] * dims[l];
+        m->b_off.push_back(m->per);
+        m->per += dims[l + 1];
+        if (dims[l + 1] > m->maxd) m->maxd = dims[l + 1];
+    }
+    if (dims[0] > m->maxd) m->maxd = dims[0];
+    const size_t P = m->per * members;
+    hipError_t e = hipGetDevice(&m->device);
+    auto A = [&](auto** ptr, size_t bytes) {
+        if (e == hipSuccess) e = hipMalloc((void**)ptr, bytes);
+    };
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreate(&m->ev0);
+    if (e == hipSuccess) e = hipEventCreate(&m->ev1);
+    A(&m->p, P * 4);
+    A(&m->pb, P * 2);
+    A(&m->grad, P * 4);
+    A(&m->am, P * 4);
+    A(&m->av, P * 4);
+    A(&m->decay, m->per);
+    m->act.assign(nlayers, nullptr);
+    m->pre.assign(nlayers, nullptr);
+    for (uint32_t l = 0; l < nlayers; l++) {
+        A(&m->act[l], (size_t)members * max_batch * dims[l] * 2);
+        A(&m->pre[l], (size_t)members * max_batch * dims[l + 1] * 4);
+    }
+    A(&m->dz[0], (size_t)members * max_batch * m->maxd * 2);
+    A(&m->dz[1], (size_t)members * max_batch * m->maxd * 2);
+    A(&m->xin, (size_t)members * max_batch * dims[0] * 4);
+    A(&m->tgt, (size_t)members * max_batch * dims[nlayers] * 4);
+    A(&m->lossm, members * 4);
+    A(&m->step, 2 * sizeof(uint32_t));
+    A(&m->loss_buf, (size_t)m->loss_cap * 4);
+    if (e == hipSuccess) {
+        std::vector<uint8_t> dec(m->per, 0);
+        for (uint32_t l = 0; l < nlayers; l++)
+            memset(dec.data() + m->w_off[l], 1, (size_t)dims[l + 1] * dims[l]);
+        e = hipMemcpy(m->decay, dec.data(), m->per, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemset(m->p, 0, P * 4);
+    if (e == hipSuccess) e = hipMemset(m->pb, 0, P * 2);
+    if (e == hipSuccess) e = hipMemset(m->am, 0, P * 4);
+    if (e == hipSuccess) e = hipMemset(m->av, 0, P * 4);
+    if (e == hipSuccess) e = hipMemset(m->step, 0, 8);
+    if (e != hipSuccess) {
+        aero_mlp_destroy(m);
+        return e == hipErrorOutOfMemory ? AERO_E_NOMEM : AERO_E_HIP;
+    }
+    *out = m;
+    return AERO_OK;
+}
+
+int aero_mlp_destroy(aero_mlp* m) {
+    if (!m) return AERO_E_INVALID;
+    (void)hipSetDevice(m->device);
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    drop_graph(m);
+    void* bufs[] = {m->p, m->pb, m->grad, m->am, m->av, m->decay, m->dz[0], m->dz[1], m->xin, m->tgt,
+                    m->lossm, m->X, m->T, m->idx, m->step, m->loss_buf};
+    for (void* b : bufs)
+        if (b) (void)hipFree(b);
+    for (auto* b : m->act)
+        if (b) (void)hipFree(b);
+    for (auto* b : m->pre)
+        if (b) (void)hipFree(b);
+    if (m->ev0) (void)hipEventDestroy(m->ev0);
+    if (m->ev1) (void)hipEventDestroy(m->ev1);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+    return AERO_OK;
+}
+
+const char* aero_mlp_last_error(const aero_mlp* m) { return m ? m->err.c_str() : "null mlp"; }
+
+uint64_t aero_mlp_param_count(const aero_mlp* m) { return m ? (uint64_t)m->per : 0u; }
+
+int aero_mlp_set_params(aero_mlp* m, const float* params) {
+    if (!m || !params) return AERO_E_INVALID;
+    MCHK(m, hipSetDevice(m->device));
+    const size_t P = m->per * m->members;
+    std::vector<uint16_t> hb(P);
+    for (size_t i = 0; i < P; i++) hb[i] = to_bf16(params[i]);
+    MCHK(m, hipMemcpyAsync(m->p, params, P * 4, hipMemcpyHostToDevice, m->stream));
+    MCHK(m, hipMemcpyAsync(m->pb, hb.data(), P * 2, hipMemcpyHostToDevice, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    return AERO_OK;
+}
+
+int aero_mlp_get_params(aero_mlp* m, float* params) {
+    if (!m || !params) return AERO_E_INVALID;
+    MCHK(m, hipSetDevice(m->device));
+    MCHK(m, hipMemcpyAsync(params, m->p, m->per * m->members * 4, hipMemcpyDeviceToHost, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    return AERO_OK;
+}
+
+int aero_mlp_reset_optimizer(aero_mlp* m) {
+    if (!m) return AERO_E_INVALID;
+    MCHK(m, hipSetDevice(m->device));
+    const size_t P = m->per * m->members;
+    MCHK(m, hipMemsetAsync(m->am, 0, P * 4, m->stream));
+    MCHK(m, hipMemsetAsync(m->av, 0, P * 4, m->stream));
+    MCHK(m, hipMemsetAsync(m->step, 0, 8, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    m->adam_t = 0;
+    return AERO_OK;
+}
+
+int aero_mlp_forward(aero_mlp* m, const float* x, uint32_t batch, float* y, float* device_ms) {
+    if (!m || !x || !y) return AERO_E_INVALID;
+    if (batch == 0 || batch > m->max_batch) return mfail(m, AERO_E_INVALID, "aero_mlp_forward: batch out of range");
+    MCHK(m, hipSetDevice(m->device));
+    const uint32_t d0 = m->dims[0], dl = m->dims[m->L];
+    const size_t nx = (size_t)batch * d0;
+    MCHK(m, hipMemcpyAsync(m->xin, x, nx * 4, hipMemcpyHostToDevice, m->stream));
+    MCHK(m, hipEventRecord(m->ev0, m->stream));
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, m->stream, m->xin, m->act[0], nx);
+    MCHK(m, hipGetLastError());
+    MCHK(m, forward(m, batch, true));
+    MCHK(m, hipEventRecord(m->ev1, m->stream));
+    // outputs of member e at pre[L-1] + e * batch * dl, contiguous
+    MCHK(m, hipMemcpyAsync(y, m->pre[m->L - 1], (size_t)m->members * batch * dl * 4, hipMemcpyDeviceToHost,
+                           m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    if (device_ms) MCHK(m, hipEventElapsedTime(device_ms, m->ev0, m->ev1));
+    return AERO_OK;
+}
+
+int aero_mlp_loss_grad(aero_mlp* m, const float* x, const float* t, uint32_t batch, float* loss, float* grads) {
+    if (!m || !x || !t) return AERO_E_INVALID;
+    if (batch == 0 || batch > m->max_batch) return mfail(m, AERO_E_INVALID, "aero_mlp_loss_grad: batch out of range");
+    MCHK(m, hipSetDevice(m->device));
+    const uint32_t d0 = m->dims[0], dl = m->dims[m->L];
+    const size_t nx = (size_t)m->members * batch * d0;
+    MCHK(m, hipMemcpyAsync(m->xin, x, nx * 4, hipMemcpyHostToDevice, m->stream));
+    MCHK(m, hipMemcpyAsync(m->tgt, t, (size_t)m->members * batch * dl * 4, hipMemcpyHostToDevice, m->stream));
+    hipLaunchKernelGGL(k_to_bf16, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, m->stream, m->xin, m->act[0], nx);
+    MCHK(m, hipGetLastError());
+    MCHK(m, forward(m, batch, false));
+    MCHK(m, launch_loss(m, batch, nullptr));
+    MCHK(m, backward(m, batch));
+    if (loss) MCHK(m, hipMemcpyAsync(loss, m->lossm, m->members * 4, hipMemcpyDeviceToHost, m->stream));
+    if (grads) MCHK(m, hipMemcpyAsync(grads, m->grad, m->per * m->members * 4, hipMemcpyDeviceToHost, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    return AERO_OK;
+}
+
+int aero_mlp_set_data(aero_mlp* m, const float* x, const float* t, uint32_t n) {
+    if (!m || !x || !t || n == 0) return AERO_E_INVALID;
+    MCHK(m, hipSetDevice(m->device));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    drop_graph(m);
+    if (m->X) (void)hipFree(m->X);
+    if (m->T) (void)hipFree(m->T);
+    m->X = m->T = nullptr;
+    m->ndata = 0;
+    MCHK(m, hipMalloc(&m->X, (size_t)n * m->dims[0] * 4));
+    MCHK(m, hipMalloc(&m->T, (size_t)n * m->dims[m->L] * 4));
+    MCHK(m, hipMemcpy(m->X, x, (size_t)n * m->dims[0] * 4, hipMemcpyHostToDevice));
+    MCHK(m, hipMemcpy(m->T, t, (size_t)n * m->dims[m->L] * 4, hipMemcpyHostToDevice));
+    m->ndata = n;
+    return AERO_OK;
+}
+
+int aero_mlp_train(aero_mlp* m, const uint32_t* idx, uint32_t nsteps, uint32_t batch, const aero_adamw* opt,
+                   float* loss_out, float* device_ms) {
+    if (!m || !idx || !opt) return AERO_E_INVALID;
+    if (!m->ndata) return mfail(m, AERO_E_STATE, "aero_mlp_train: no data (aero_mlp_set_data)");
+    if (batch == 0 || batch > m->max_batch) return mfail(m, AERO_E_INVALID, "aero_mlp_train: batch out of range");
+    if (nsteps == 0 || nsteps > m->loss_cap) return mfail(m, AERO_E_INVALID, "aero_mlp_train: nsteps out of range");
+    const size_t nidx = (size_t)nsteps * m->members * batch;
+    for (size_t i = 0; i < nidx; i++)
+        if (idx[i] >= m->ndata) return mfail(m, AERO_E_INVALID, "aero_mlp_train: sample index out of range");
+    MCHK(m, hipSetDevice(m->device));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    if (nidx > m->idx_cap) {
+        drop_graph(m);  // the graph holds the old index buffer
+        if (m->idx) (void)hipFree(m->idx);
+        m->idx = nullptr;
+        m->idx_cap = 0;
+        MCHK(m, hipMalloc(&m->idx, nidx * 4));
+        m->idx_cap = nidx;
+    }
+    MCHK(m, hipMemcpyAsync(m->idx, idx, nidx * 4, hipMemcpyHostToDevice, m->stream));
+    // training-step counter restarts at 0 for this call; the optimizer count carries on
+    MCHK(m, hipMemsetAsync(m->step + 1, 0, 4, m->stream));
+    const bool same = m->gexec && m->g_batch == batch && memcmp(&m->g_opt, opt, sizeof(aero_adamw)) == 0;
+    if (!same) {
+        drop_graph(m);
+        hipGraph_t g = nullptr;
+        MCHK(m, hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = train_step(m, batch, *opt);
+        hipError_t e2 = hipStreamEndCapture(m->stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return mhip(m, e != hipSuccess ? e : e2, "aero_mlp_train: capture");
+        }
+        e = hipGraphInstantiate(&m->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) return mhip(m, e, "hipGraphInstantiate");
+        m->g_batch = batch;
+        m->g_opt = *opt;
+    }
+    MCHK(m, hipEventRecord(m->ev0, m->stream));
+    for (uint32_t s = 0; s < nsteps; s++) MCHK(m, hipGraphLaunch(m->gexec, m->stream));
+    MCHK(m, hipEventRecord(m->ev1, m->stream));
+    if (loss_out) MCHK(m, hipMemcpyAsync(loss_out, m->loss_buf, (size_t)nsteps * 4, hipMemcpyDeviceToHost, m->stream));
+    MCHK(m, hipStreamSynchronize(m->stream));
+    if (device_ms) MCHK(m, hipEventElapsedTime(device_ms, m->ev0, m->ev1));
+    m->adam_t += nsteps;
+    return AERO_OK;
+}
+
+}  // extern "C"

@@ -212,6 +212,41 @@ int aero_batch_autotune(aero_batch* batch, uint32_t steps_per_shape, int* shape,
 int aero_batch_set_launch_shape(aero_batch* batch, int shape);
 int aero_batch_get_launch_shape(aero_batch* batch, int* shape);
 
+/* ---- Surrogate MLP ensemble ----
+ * No counterpart in the reference: there every setup-to-force answer is an LBM run
+ * (the aero-map sweep, src/ui/aeromap.ts:73-114; the garage balance,
+ * src/ui/garage.ts:124).  An ensemble of `members` networks with widths
+ * dims[0] -> ... -> dims[nlayers], GELU (erf form) after every hidden layer, BF16
+ * operands on MFMA with FP32 accumulation, FP32 master weights and AdamW moments.
+ * Parameters of one member: for each layer, W (dims[l+1] x dims[l], row-major) then
+ * b (dims[l+1]); members follow one another. */
+typedef struct aero_mlp aero_mlp;
+typedef struct {
+    float lr, beta1, beta2, eps, weight_decay; /* decoupled decay, weights only */
+    uint32_t pad[3];
+} aero_adamw;
+int aero_mlp_create(uint32_t members, uint32_t nlayers, const uint32_t* dims, uint32_t max_batch, aero_mlp** out);
+int aero_mlp_destroy(aero_mlp* mlp);
+const char* aero_mlp_last_error(const aero_mlp* mlp);
+uint64_t aero_mlp_param_count(const aero_mlp* mlp); /* per member */
+int aero_mlp_set_params(aero_mlp* mlp, const float* params);
+int aero_mlp_get_params(aero_mlp* mlp, float* params);
+int aero_mlp_reset_optimizer(aero_mlp* mlp);
+/* Every member on the same inputs x [batch x dims[0]] -> y [members x batch x dims[L]].
+ * *device_ms (nullable): device time of the network (input conversion + layers). */
+int aero_mlp_forward(aero_mlp* mlp, const float* x, uint32_t batch, float* y, float* device_ms);
+/* Mean-squared-error loss per member and its gradient with respect to every
+ * parameter, each member on its own inputs: x [members x batch x dims[0]],
+ * t [members x batch x dims[L]] -> loss [members], grads [members x param_count]. */
+int aero_mlp_loss_grad(aero_mlp* mlp, const float* x, const float* t, uint32_t batch, float* loss, float* grads);
+/* Training set kept on the device: x [n x dims[0]], t [n x dims[L]]. */
+int aero_mlp_set_data(aero_mlp* mlp, const float* x, const float* t, uint32_t n);
+/* nsteps AdamW steps replayed from a captured graph; idx [nsteps x members x batch]
+ * holds each member's sample indices per step (bootstrap draws are the caller's).
+ * loss_out [nsteps] (nullable): mean over members of the step's loss. */
+int aero_mlp_train(aero_mlp* mlp, const uint32_t* idx, uint32_t nsteps, uint32_t batch, const aero_adamw* opt,
+                   float* loss_out, float* device_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -96,3 +96,18 @@ def test_batch_entry_points_reject_bad_arguments_before_any_gpu_call():
     assert lib.aero_batch_instance(None, 0) is None
     assert lib.aero_batch_size(None) == 0
     assert lib.aero_batch_last_error(None) == b"null batch"
+
+
+def test_mlp_entry_points_reject_bad_arguments_before_any_gpu_call():
+    lib = _lib.load()
+    h = C.c_void_p()
+    dims = (C.c_uint32 * 3)(4, 8, 2)
+    assert lib.aero_mlp_create(0, 2, dims, 16, C.byref(h)) == -1
+    assert lib.aero_mlp_create(1, 0, dims, 16, C.byref(h)) == -1
+    assert lib.aero_mlp_create(1, 2, dims, 0, C.byref(h)) == -1
+    bad = (C.c_uint32 * 3)(4, 0, 2)
+    assert lib.aero_mlp_create(1, 2, bad, 16, C.byref(h)) == -1
+    assert lib.aero_mlp_forward(None, None, 1, None, None) == -1
+    assert lib.aero_mlp_train(None, None, 1, 1, None, None, None) == -1
+    assert lib.aero_mlp_param_count(None) == 0
+    assert lib.aero_mlp_last_error(None) == b"null mlp"

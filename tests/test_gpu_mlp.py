@@ -1,0 +1,108 @@
+"""The surrogate network's GPU kernels (BF16 MFMA forward, backward, AdamW) against the
+FP64 oracle of oracle/mlp_oracle.py.
+
+Tolerances: against the oracle with BF16 rounding at the kernels' storage points the
+only differences are FP32 accumulation order and the rare BF16 rounding flip it
+causes: relative L2 <= 2e-3 per member (outputs and gradients).  Against the plain
+FP64 oracle the BF16 operands themselves count: relative L2 <= 5e-2."""
+import numpy as np
+import pytest
+
+from aerolbm.mlp import SURROGATE_DIMS, MLPEnsemble
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("batch", [64, 300])
+def test_forward_matches_oracle(batch):
+    print(f"mlp forward batch={batch}", flush=True)
+    net = MLPEnsemble(SURROGATE_DIMS, members=3, max_batch=512, seed=1)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((batch, 16)).astype(np.float32)
+    y, ms = net.forward(x)
+    p = net.params().astype(np.float64)
+    for m in range(3):
+        ybf, _, _ = mo.forward(p[m], SURROGATE_DIMS, x, bf=True)
+        y64, _, _ = mo.forward(p[m], SURROGATE_DIMS, x)
+        assert rel(y[m], ybf) <= 2e-3
+        assert rel(y[m], y64) <= 5e-2
+    assert ms > 0
+    net.close()
+
+
+def test_loss_gradient_matches_oracle():
+    """Gradient check: every parameter's gradient from the GPU backward kernels
+    against the oracle's, member by member, on each member's own inputs."""
+    print("mlp gradient check", flush=True)
+    net = MLPEnsemble(SURROGATE_DIMS, members=2, max_batch=256, seed=3)
+    rng = np.random.default_rng(4)
+    B = 200
+    x = rng.standard_normal((2, B, 16)).astype(np.float32)
+    t = rng.standard_normal((2, B, 4)).astype(np.float32)
+    loss, g = net.loss_grad(x, t)
+    p = net.params().astype(np.float64)
+    for m in range(2):
+        lbf, gbf = mo.loss_grad(p[m], SURROGATE_DIMS, x[m], t[m], bf=True)
+        l64, g64 = mo.loss_grad(p[m], SURROGATE_DIMS, x[m], t[m])
+        print(f"  member {m}: loss {loss[m]:.6f} vs {lbf:.6f}; grad rel err vs bf16 oracle {rel(g[m], gbf):.2e}, "
+              f"vs fp64 {rel(g[m], g64):.2e}", flush=True)
+        assert abs(loss[m] - lbf) <= 2e-3 * lbf
+        assert rel(g[m], gbf) <= 2e-3
+        assert rel(g[m], g64) <= 5e-2
+        # per layer, so that a wrong small layer cannot hide under a large one
+        o = 0
+        for l in range(len(SURROGATE_DIMS) - 1):
+            n, k = SURROGATE_DIMS[l + 1], SURROGATE_DIMS[l]
+            for size in (n * k, n):
+                assert rel(g[m][o:o + size], gbf[o:o + size]) <= 5e-3
+                o += size
+
+
+def test_training_fits_and_replays_deterministically():
+    """AdamW from the captured training graph: the loss of a smooth target falls by
+    >20x, two runs from the same start and indices give identical parameters and step
+    losses, and the trained network's forward still matches the oracle."""
+    print("mlp training", flush=True)
+    rng = np.random.default_rng(5)
+    X = rng.uniform(-1, 1, (2048, 16)).astype(np.float32)
+    T = np.stack([np.sin(2 * X[:, 0]) + X[:, 1] * X[:, 2], np.cos(X[:, 3]) - X[:, 4] ** 2,
+                  X[:, 5] + 0.5 * X[:, 6] * X[:, 7], np.tanh(X[:, 8] - X[:, 9])], 1).astype(np.float32)
+    runs = []
+    for rep in range(2):
+        net = MLPEnsemble(SURROGATE_DIMS, members=2, max_batch=256, seed=7)
+        net.set_data(X, T)
+        idx = np.random.default_rng(8).integers(0, 2048, (600, 2, 128)).astype(np.uint32)
+        loss, ms = net.train(600, batch=128, lr=2e-3, idx=idx)
+        runs.append((net.params(), loss))
+        print(f"  run {rep}: loss {loss[:5].mean():.4f} -> {loss[-20:].mean():.5f}, {ms:.1f} ms device", flush=True)
+        if rep == 0:
+            first = net
+        else:
+            net.close()
+    assert np.array_equal(runs[0][0], runs[1][0]) and np.array_equal(runs[0][1], runs[1][1])
+    loss = runs[0][1]
+    assert loss[-20:].mean() < loss[:5].mean() / 20
+    # full-set loss after training from the forward kernels agrees with the oracle
+    y, _ = first.forward(X[:256])
+    p = first.params().astype(np.float64)
+    ybf, _, _ = mo.forward(p[0], SURROGATE_DIMS, X[:256], bf=True)
+    assert rel(y[0], ybf) <= 2e-3
+    first.close()
+
+
+def test_bad_arguments_raise():
+    import aerolbm
+    net = MLPEnsemble((4, 8, 2), members=1, max_batch=16)
+    with pytest.raises(aerolbm.AeroError):
+        net.forward(np.zeros((17, 4), np.float32))  # above max_batch
+    with pytest.raises(aerolbm.AeroError):
+        net.train(10, batch=8, idx=np.zeros((10, 1, 8), np.uint32))  # no data yet
+    net.set_data(np.zeros((10, 4), np.float32), np.zeros((10, 2), np.float32))
+    with pytest.raises(aerolbm.AeroError):
+        net.train(2, batch=4, idx=np.full((2, 1, 4), 10, np.uint32))  # index out of range
+    net.close()
