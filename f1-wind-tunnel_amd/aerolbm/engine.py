@@ -45,21 +45,22 @@ class ShapeKind:
 
 @dataclass
 class ShapeParams:
-    """src/engine/engine.ts:36-65 (simulation-relevant fields)."""
+    """src/engine/engine.ts:36-65 (simulation-relevant fields); defaults are the app's
+    initial setup (src/main.ts:99-100)."""
     kind: int = ShapeKind.F1
     frontWing: int = 0
     rearWing: int = 0
     sidepod: int = 0
     cover: int = 0
     nose: int = 0
-    rearRideH: float = 0.12
+    rearRideH: float = 0.14
     rearMainDeg: float = 0.0
     frontFlapDeg: float = 0.0
     rearFlapDeg: float = 0.0
-    rideH: float = 0.10
+    rideH: float = 0.12
     aoaDeg: float = 6.0
     customMode: int = 0
-    customWidth: float = 0.5
+    customWidth: float = 0.6
     customAxisY: float = 1.0
     heatBrakes: bool = True
     heatEngine: bool = True

@@ -1,0 +1,265 @@
+"""Setup -> force-coefficient surrogate: an MLP ensemble trained on batched LBM runs of
+the generic F1 body, with the LBM as fallback and judge.
+
+Inputs (16, the setup of one run):
+    0 front ride height (m, 0.020-0.060)        1 rear ride height (m, 0.040-0.120)
+    2-4 front wing main / flap 1 / flap 2 angle (deg, 4-14; each flap measured from
+        the element before it)
+    5-6 rear wing main / flap angle (deg, 4-14; flap from main)   7 beam wing (deg, 4-14)
+    8-11 sidepod ("pontoon") variant, one-hot of 4
+    12-14 cooling level (engine-cover hump), one-hot of 3
+    15 gurney flap on the rear-wing trailing edge (0/1)
+Outputs (4): Cl*A of the front-wing group, Cl*A of the rear-wing group (downforce
+positive), Cd*A of the whole car, and balance = front / (|front| + |rear|) wing
+downforce: the garage's fallback balance front / (front + rear)
+(src/ui/garage.ts:124) whenever both wings make downforce, and bounded in [-1, 1]
+when one of them lifts.  Coefficients are per unit dynamic pressure for the full
+(mirrored) car.
+
+Runs: the 320x96x96 half tunnel at 4.5 cm cells (the generic F1 workload's grid and
+lattice parameters), FP16-shifted storage, all setups of a chunk in one AeroBatch,
+``settle`` steps then forces averaged over ``measure`` steps.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+
+from . import workloads as W
+from .batch import AeroBatch
+from .mlp import SURROGATE_DIMS, MLPEnsemble
+
+DEG = math.pi / 180
+N_IN, N_OUT = 16, 4
+OUT_NAMES = ("ClA_front", "ClA_rear", "CdA", "balance")
+CONT_LO = np.array([0.020, 0.040, 4, 4, 4, 4, 4, 4], np.float64)
+CONT_HI = np.array([0.060, 0.120, 14, 14, 14, 14, 14, 14], np.float64)
+COOLING_HUMP = (0.05, 0.12, 0.20)
+
+
+def f1_grid() -> dict:
+    w = W.f1()
+    return w.engine_kwargs()
+
+
+def lhs_setups(n: int, seed: int = 0) -> np.ndarray:
+    """Latin-hypercube design: every continuous input and every categorical one (as a
+    uniform variable binned into its levels) is stratified into n equal bins."""
+    rng = np.random.default_rng(seed)
+
+    def strat():
+        return (rng.permutation(n) + rng.uniform(0, 1, n)) / n
+
+    X = np.zeros((n, N_IN), np.float64)
+    for k in range(8):
+        X[:, k] = CONT_LO[k] + strat() * (CONT_HI[k] - CONT_LO[k])
+    pod = np.minimum((strat() * 4).astype(int), 3)
+    cool = np.minimum((strat() * 3).astype(int), 2)
+    X[np.arange(n), 8 + pod] = 1.0
+    X[np.arange(n), 12 + cool] = 1.0
+    X[:, 15] = (strat() < 0.5).astype(np.float64)
+    return X
+
+
+def _element_chain(le, chord, ang, material, thick, z0, z1, gap=(0.03, 0.025)):
+    """Wing prims of a multi-element wing: each element starts just behind and above
+    the trailing edge of the one before."""
+    P, x, y = [], le[0], le[1]
+    for c, a in zip(chord, ang):
+        P.append(W.prim(W.PRIM_WING, material, 1, x, y, c, a, thick * c / chord[0], z0, z1))
+        x, y = x + math.cos(a) * c - gap[0], y + math.sin(a) * c + gap[1]
+    return P, (x + gap[0], y - gap[1])
+
+
+def setup_scene(s: np.ndarray):
+    """(Shape, prims) of one setup: the generic F1 body (rounded-box chassis 5.0 x 1.9
+    x 0.95 m, wheels r = 0.36 m w = 0.30 m) with the setup's wings and parts, raked by
+    the ride heights about the front axle."""
+    fr, rr = float(s[0]), float(s[1])
+    fa = np.cumsum(s[2:5]) * DEG
+    ra = np.cumsum(s[5:7]) * DEG
+    beam = float(s[7]) * DEG
+    pod = int(np.argmax(s[8:12]))
+    cool = int(np.argmax(s[12:15]))
+    gurney = s[15] > 0.5
+    b = fr
+    P = [W.prim(W.PRIM_ROUNDBOX, W.MAT_REST, 1, 2.95, b + 0.475, 0.0, 2.5, 0.475, 0.95, 0.08),
+         W.prim(W.PRIM_WHEEL, W.MAT_REST, 0, 1.0, 0.36, 1.12, 0.36, 0.15),
+         W.prim(W.PRIM_WHEEL, W.MAT_REST, 0, 4.4, 0.36, 1.12, 0.36, 0.15)]
+    fw, _ = _element_chain((-0.40, b + 0.09), (0.30, 0.18, 0.12), fa, W.MAT_FRONT_WING, 0.028, -0.9, 0.9)
+    rw, te = _element_chain((5.0, b + 1.12), (0.34, 0.22), ra, W.MAT_REAR_WING, 0.04, -0.5, 0.5)
+    P += fw + rw
+    P.append(W.prim(W.PRIM_WING, W.MAT_REAR_WING, 1, 5.05, b + 0.62, 0.22, beam, 0.03, -0.5, 0.5))
+    P.append(W.prim(W.PRIM_ROUNDBOX, W.MAT_REST, 1, 5.15, b + 1.05, 0.0, 0.06, 0.12, 0.02, 0.01))  # pylon
+    pods = [None, (2.4, 1.05, 0.55, 0.9, 0.12, 0.35), (2.9, 1.0, 0.6, 1.2, 0.08, 0.3),
+            (2.2, 1.1, 0.5, 0.6, 0.18, 0.4)]
+    if pods[pod]:
+        cx, cy, cz, hx, hy, hz = pods[pod]
+        P.append(W.prim(W.PRIM_ROUNDBOX, W.MAT_REST, 1, cx, b + cy, cz, hx, hy, hz, 0.06))
+    hump = COOLING_HUMP[cool]
+    P.append(W.prim(W.PRIM_ROUNDBOX, W.MAT_REST, 1, 3.6, b + 0.95 + hump / 2, 0.0, 0.8, hump / 2 + 0.02, 0.25, 0.04))
+    if gurney:
+        P.append(W.prim(W.PRIM_ROUNDBOX, W.MAT_REAR_WING, 1, te[0] - 0.01, te[1] + 0.03, 0.0, 0.012, 0.035, 0.5,
+                        0.004))
+    return W.base_shape(W.ShapeKind.Scene, ride_h=fr, rear_ride=rr), P
+
+
+def coefficients(sums: np.ndarray, steps: int, h: float, u: float) -> np.ndarray:
+    k = 4.0 * h * h / (u * u) / steps
+    fx = float(sums[0] + sums[3] + sums[6])
+    cl_f, cl_r = -k * float(sums[4]), -k * float(sums[7])
+    return np.array([cl_f, cl_r, k * fx, balance(cl_f, cl_r)])
+
+
+def balance(cl_f, cl_r):
+    den = np.abs(cl_f) + np.abs(cl_r)
+    return np.where(den > 1e-12, cl_f / np.where(den > 1e-12, den, 1.0), 0.5)
+
+
+DATA = Path(__file__).resolve().parent / "data" / "f1_setups_2048.npz"
+
+
+def load_dataset(path: Path = DATA) -> Dict[str, np.ndarray]:
+    """The committed training set (see main()); balance is recomputed from the two
+    wing coefficients."""
+    with np.load(path, allow_pickle=False) as d:
+        out = {k: d[k] for k in d.files}
+    for k in ("Y", "Y_repeat"):
+        y = out[k].astype(np.float64)
+        y[:, 3] = balance(y[:, 0], y[:, 1])
+        out[k] = y
+    out["X"] = out["X"].astype(np.float64)
+    return out
+
+
+def run_lbm(setups: np.ndarray, settle: int, measure: int, chunk: int = 64, storage: str = "f16s",
+            perturb: float = 0.0, seed: int = 0, log=None) -> Tuple[np.ndarray, float]:
+    """Labels of every setup from batched LBM runs; returns (Y [n, 4], wall seconds)."""
+    g = f1_grid()
+    Y = np.zeros((len(setups), N_OUT))
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    for c0 in range(0, len(setups), chunk):
+        part = setups[c0:c0 + chunk]
+        b = AeroBatch([g] * len(part), storage=storage)
+        try:
+            b.set_launch_shape(0)
+            for e, s in zip(b.engines, part):
+                shape, prims = setup_scene(s)
+                e.set_shape(shape, prims=prims)
+                e.reset()
+                if perturb:
+                    p = e.populations()
+                    e.write_populations((p * (1 + perturb * rng.standard_normal(p.shape))).astype(np.float32))
+            b.run_graph(settle, 100)
+            for e in b.engines:
+                e.read_forces_raw(reset=True)
+            b.run_graph(measure, 100)
+            for i, e in enumerate(b.engines):
+                d, _, n = e.read_forces_raw(reset=True)
+                Y[c0 + i] = coefficients(d, n, e.h, e.u_lb)
+        finally:
+            b.close()
+        if log:
+            log(f"  LBM {c0 + len(part)}/{len(setups)} setups, {time.perf_counter() - t0:.1f} s")
+    return Y, time.perf_counter() - t0
+
+
+@dataclass
+class Answer:
+    y: np.ndarray            # [n, 4]
+    from_surrogate: np.ndarray  # [n] bool
+    spread: np.ndarray       # [n] ensemble spread (relative, max over outputs)
+    surrogate_ms: float
+    lbm_s: float
+
+
+class Surrogate:
+    """Normalised-input, normalised-output MLP ensemble (5 members by default)."""
+
+    def __init__(self, members: int = 5, seed: int = 0, max_batch: int = 4096):
+        self.net = MLPEnsemble(SURROGATE_DIMS, members=members, max_batch=max_batch, seed=seed)
+        self.rng = np.random.default_rng(seed + 1)
+
+    def fit(self, X: np.ndarray, Y: np.ndarray, val: Optional[Tuple[np.ndarray, np.ndarray]] = None,
+            steps: int = 4000, batch: int = 256, lr: float = 1e-3, weight_decay: float = 1e-2,
+            check_every: int = 200) -> Dict[str, np.ndarray]:
+        """AdamW on the GPU, every member on its own bootstrap resample.  With ``val``
+        the parameters of the step with the lowest validation loss (ensemble mean, in
+        normalised units) are kept; the learning rate drops tenfold for the last
+        quarter of the steps.  Returns the training and validation loss curves."""
+        self.xm, self.xs = X.mean(0), X.std(0) + 1e-9
+        self.ym, self.ys = Y.mean(0), Y.std(0) + 1e-9
+        self.floor = 0.25 * self.ys  # spread denominators never fall below a quarter of the label std
+        self.net.set_data((X - self.xm) / self.xs, (Y - self.ym) / self.ys)
+        idx = self.net.bootstrap_indices(steps, batch)
+        train_loss, val_loss = [], []
+        best, best_p = np.inf, None
+        done = 0
+        while done < steps:
+            k = min(check_every, steps - done)
+            rate = lr if done < 0.75 * steps else lr / 10
+            loss, _ = self.net.train(k, batch=batch, lr=rate, weight_decay=weight_decay, idx=idx[done:done + k])
+            train_loss.append(loss)
+            done += k
+            if val is not None:
+                mean, _, _ = self.predict(val[0])
+                v = float(np.mean(((mean - val[1]) / self.ys) ** 2))
+                val_loss.append(v)
+                if v < best:
+                    best, best_p = v, self.net.params()
+        if best_p is not None:
+            self.net.set_params(best_p)
+        return {"train": np.concatenate(train_loss), "val": np.array(val_loss)}
+
+    def predict(self, X: np.ndarray):
+        """Returns (mean [n, 4], spread [n], device ms)."""
+        y, ms = self.net.forward((X - self.xm) / self.xs)
+        y = y.astype(np.float64) * self.ys + self.ym  # [members, n, 4]
+        mean = y.mean(0)
+        spread = (y.std(0) / np.maximum(np.abs(mean), self.floor)).max(1)
+        return mean, spread, ms
+
+    def answer(self, X: np.ndarray, settle: int, measure: int, max_spread: float = 0.02, log=None) -> Answer:
+        """Surrogate where the ensemble agrees within max_spread (the largest over the
+        four outputs of member std / max(|mean|, floor)), batched LBM for the rest."""
+        mean, spread, ms = self.predict(X)
+        ok = spread <= max_spread
+        y = mean.copy()
+        lbm_s = 0.0
+        if (~ok).any():
+            y[~ok], lbm_s = run_lbm(X[~ok], settle, measure, log=log)
+        return Answer(y, ok, spread, ms, lbm_s)
+
+    def close(self) -> None:
+        self.net.close()
+
+
+def main() -> None:
+    """python -m aerolbm.surrogate OUT.npz [n] [settle] [measure]: writes the training
+    set (LHS setups and their LBM labels, plus repeat runs of the first 64 setups from
+    a perturbed start, which measure how reproducible the labels themselves are)."""
+    import sys
+
+    out = sys.argv[1]
+    n = int(sys.argv[2]) if len(sys.argv) > 2 else 2048
+    settle = int(sys.argv[3]) if len(sys.argv) > 3 else 2400
+    measure = int(sys.argv[4]) if len(sys.argv) > 4 else 2400
+    log = lambda s: print(s, flush=True)
+    X = lhs_setups(n, seed=0)
+    log(f"generating {n} setups, {settle} + {measure} steps each")
+    Y, wall = run_lbm(X, settle, measure, log=log)
+    rep = min(64, n)
+    Yr, wall_r = run_lbm(X[:rep], settle, measure, perturb=1e-3, seed=1, log=log)
+    np.savez_compressed(out, X=X.astype(np.float32), Y=Y.astype(np.float32), Y_repeat=Yr.astype(np.float32),
+                        settle=settle, measure=measure, wall_s=wall, cells=320 * 96 * 96)
+    log(f"wrote {out}: {wall:.1f} s for {n} runs")
+
+
+if __name__ == "__main__":
+    main()

@@ -2,7 +2,8 @@
 
     python f1-wind-tunnel_amd/build.py
 
-Output: f1-wind-tunnel_amd/lib/libaero_lbm.so.  The library is compiled with
+Output: f1-wind-tunnel_amd/lib/libaero_lbm.so and, where Node's headers are
+installed, the N-API addon lib/aero_napi.node over it.  The library is compiled with
 -ffp-contract=off (host and device) so that the f32 collision keeps the
 reference's operation order, and with packed-FP32 instructions disabled.
 """
@@ -60,6 +61,24 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB / "libaero_lbm.so.tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
     os.replace(tmp, out)
+    if verbose:
+        print(f"built {out}")
+    build_addon(verbose)
+    return out
+
+
+NODE_INCLUDE = Path(os.environ.get("NODE_INCLUDE", "/usr/include/node"))
+
+
+def build_addon(verbose: bool = False) -> Path | None:
+    """aero_napi.node: the N-API binding, linked to libaero_lbm.so next to it."""
+    if not (NODE_INCLUDE / "node_api.h").exists():
+        if verbose:
+            print(f"skipped the N-API addon: {NODE_INCLUDE}/node_api.h not found")
+        return None
+    out = LIB / "aero_napi.node"
+    _run(["gcc", "-shared", "-fPIC", "-O2", "-std=c11", "-Wall", f"-I{NODE_INCLUDE}", f"-I{PKG.parent / 'include'}",
+          str(PKG / "node" / "aero_napi.c"), "-o", str(out), f"-L{LIB}", "-laero_lbm", "-Wl,-rpath,$ORIGIN"])
     if verbose:
         print(f"built {out}")
     return out

@@ -1,0 +1,41 @@
+"""The TypeScript-facing host (f1-wind-tunnel_amd/node: N-API addon + HipAeroEngine) on
+the GPU: its own checks (closed-form cube voxel count, uniform-flow fixed point,
+force scaling, frame cadence, error mapping), and its cube forces against the Python
+host driving the same library with the same inputs."""
+import json
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import aerolbm
+from aerolbm import workloads as W
+
+ROOT = Path(__file__).resolve().parent.parent
+NODE_DIR = ROOT / "f1-wind-tunnel_amd" / "node"
+ADDON = ROOT / "f1-wind-tunnel_amd" / "lib" / "aero_napi.node"
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(shutil.which("node") is None or not ADDON.exists(),
+                                 reason="node or the built addon is absent")]
+
+
+def test_node_engine_checks_and_python_agreement():
+    print("node test_hip_engine.js", flush=True)
+    r = subprocess.run(["node", "test_hip_engine.js"], cwd=NODE_DIR, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert all(v["ok"] for k, v in out.items() if isinstance(v, dict) and "ok" in v)
+    # the same 200 cube steps through the Python host: identical FP64 force sums
+    wl = W.cube()
+    e = aerolbm.HipAeroEngine(math="exact", **wl.engine_kwargs())
+    e.set_shape(wl.shape)
+    e.reset()
+    e.step(200)
+    f = e.read_forces(reset=True)
+    js = out["cube_forces"]
+    assert f.dragN == pytest.approx(js["dragN"], rel=1e-12)
+    assert f.downforceN == pytest.approx(js["downforceN"], rel=1e-12, abs=1e-12)
