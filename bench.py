@@ -113,6 +113,40 @@ def aeromap_bench(steps: int = 2000) -> dict:
     }
 
 
+def surrogate_bench() -> dict:
+    """The learned part: 5-member ensemble trained on the GPU on the committed 2048-run
+    training set, then 4096 setups answered; the LBM cost per setup is the batched
+    generation time of that set."""
+    import numpy as np
+
+    from aerolbm import surrogate as S
+
+    d = S.load_dataset()
+    X, Y = d["X"], d["Y"]
+    test, val, train = np.arange(0, 256), np.arange(256, 512), np.arange(512, len(X))
+    print("[bench] surrogate: training 5 members", file=sys.stderr, flush=True)
+    sur = S.Surrogate(members=5, seed=0)
+    t0 = time.perf_counter()
+    sur.fit(X[train], Y[train], val=(X[val], Y[val]), steps=3000)
+    train_s = time.perf_counter() - t0
+    mean, _, _ = sur.predict(X[test])
+    r2 = 1.0 - ((mean - Y[test]) ** 2).mean(0) / Y[test].var(0)
+    q = S.lhs_setups(4096, seed=7)
+    sur.predict(q)  # first call warms the kernels
+    _, spread, ms = sur.predict(q)
+    sur.close()
+    lbm_s = float(d["wall_s"]) / len(X)
+    return {
+        "members": 5, "params_per_member": 202756, "train_setups": int(len(train)), "train_steps": 3000,
+        "train_s": round(train_s, 2), "inference_batch": 4096, "inference_ms": round(ms, 3),
+        "lbm_s_per_setup": round(lbm_s, 4), "gain_per_answer": float(f"{lbm_s * 4096 / (ms / 1e3):.3g}"),
+        "r2_test": {k: round(float(v), 3) for k, v in zip(S.OUT_NAMES, r2)},
+        "accepted_at_2pct_spread": float((spread <= 0.02).mean()),
+        "label_repeat_scatter_median": {k: round(float(v), 4) for k, v in
+                                        zip(S.OUT_NAMES, np.median(np.abs(d["Y_repeat"] - Y[:64]), 0))},
+    }
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +157,7 @@ def main() -> None:
     ap.add_argument("--math", default="fast", choices=["fast", "exact"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-aeromap", action="store_true")
+    ap.add_argument("--no-surrogate", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -224,6 +259,8 @@ def main() -> None:
     e.close()
     if not args.no_aeromap:
         result["aeromap"] = aeromap_bench()
+    if not args.no_surrogate:
+        result["surrogate"] = surrogate_bench()
     if not args.no_cpu:
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(wl)

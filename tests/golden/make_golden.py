@@ -1,0 +1,122 @@
+"""Writes the golden fixtures of this directory from the CPU oracle, and their SHA-256
+manifest.  The reference ships no known-answer vectors for its kernels (its physics is
+WGSL run by WebGPU), so these fixtures are the oracle's answers on small seeded cases;
+tests/test_golden.py pins the oracle to them and the GPU tests compare the library
+with them.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+for p in (ROOT, ROOT / "f1-wind-tunnel_amd"):
+    sys.path.insert(0, str(p))
+
+from oracle import oracle as orc  # noqa: E402
+
+DIMS = (24, 12, 8)
+H = 0.1
+
+
+def mixed_geometry():
+    """Solids of all three groups, a floor-touching block, a body on the symmetry
+    plane, duct cells and two heat sources on a 24 x 12 x 8 grid."""
+    nx, ny, nz = DIMS
+    s = np.zeros((nz, ny, nx), np.uint32)
+    e = np.zeros((nz, ny, nx), np.float32)
+    s[0:3, 3:6, 5:8] = 1
+    s[2:4, 6:8, 10:13] = 2
+    s[4:6, 7:9, 15:17] = 3
+    s[2:4, 0:2, 18:20] = 1
+    e[5:7, 3:5, 8:12] = -1.0
+    e[1, 4, 6] = 420.0
+    e[3, 7, 11] = 150.0
+    return s.reshape(-1), e.reshape(-1)
+
+
+def lbm_fixture():
+    g = orc.grid(DIMS, H, (0.0, 0.0, 0.0))
+    sim = orc.OSim(0.06, 0.51, 0.1, 0.0)
+    F32 = orc.LBMOracle("f32")
+    F64 = orc.LBMOracle("f64")
+    solid, emit = mixed_geometry()
+    f0, fields0 = F32.init(g, sim)
+    rng = np.random.default_rng(10)
+    f = (f0 * (1 + 0.02 * rng.standard_normal(f0.shape))).astype(np.float32)
+    out = dict(solid=solid, emit=emit, f0=f, fields0=fields0, dims=np.array(DIMS), h=np.float32(H),
+               sim=np.array([0.06, 0.51, 0.1], np.float32))
+    for steps in (1, 100):
+        fo, fl, st = F32.run(g, sim, f, solid, fields0, emit, steps)
+        out[f"f_{steps}"] = fo
+        out[f"fields_{steps}"] = fl
+        out[f"forces_q_{steps}"] = np.array(st.forces_q, np.int64)
+        out[f"forces_d_{steps}"] = np.array(st.forces_d, np.float64)
+        out[f"guard_{steps}"] = np.uint64(st.guard_hits)
+        _, _, st64 = F64.run(g, sim, f.astype(np.float64), solid, fields0.astype(np.float64), emit, steps)
+        out[f"forces_d64_{steps}"] = np.array(st64.forces_d, np.float64)
+    return out
+
+
+def heat_fixture(lbm):
+    g = orc.grid(DIMS, H, (0.0, 0.0, 0.0))
+    F32 = orc.LBMOracle("f32")
+    rng = np.random.default_rng(11)
+    tin = rng.uniform(0.0, 300.0, g.n).astype(np.float32)
+    hp = orc.OHeat(4.0, 0.004, 0.04, 0.0)
+    tout = F32.heat(g, hp, lbm["fields_100"], lbm["emit"], tin)
+    return dict(fields=lbm["fields_100"], emit=lbm["emit"], tin=tin, tout=tout, heat=np.array([4.0, 0.004, 0.04], np.float32))
+
+
+def shape(kind, **kw):
+    s = orc.OShape()
+    s.kind = kind
+    s.pw, s.ph, s.pScale = 256, 128, 0.02
+    s.rideH, s.rearRide, s.aoa = kw.get("rideH", 0.10), kw.get("rearRide", 0.12), kw.get("aoa", 6 * math.pi / 180)
+    s.frontFlap, s.rearMain, s.rearFlap = 14 * math.pi / 180, 9 * math.pi / 180, 32 * math.pi / 180
+    s.heatBrakes, s.heatEngine, s.heatRad = 520.0, 620.0, 75.0
+    s.duct = 1.0
+    return s
+
+
+def vox_fixture():
+    SDF = orc.SDFOracle()
+    out = {}
+    cases = {
+        "sphere": ((64, 32, 24), 0.08, (0.0, 0.0, 0.0), shape(1)),
+        "cube": ((64, 32, 24), 0.08, (0.0, 0.0, 0.0), shape(2)),
+        "wing": ((64, 32, 24), 0.08, (0.0, 0.0, 0.0), shape(3)),
+        "f1": ((116, 34, 26), 10.4 / 116, (-2.4, 0.0, 0.0), shape(0)),
+    }
+    for name, (dims, h, origin, s) in cases.items():
+        g = orc.grid(dims, h, origin)
+        solid, emit, sdf = SDF.voxelize(g, s)
+        out[f"{name}_dims"] = np.array(dims)
+        out[f"{name}_h"] = np.float32(h)
+        out[f"{name}_origin"] = np.array(origin, np.float32)
+        out[f"{name}_solid"] = solid.astype(np.uint8)
+        out[f"{name}_emit"] = emit
+        out[f"{name}_sdf"] = sdf
+    return out
+
+
+def main() -> None:
+    lbm = lbm_fixture()
+    files = {"lbm_mixed_24x12x8.npz": lbm, "heat_24x12x8.npz": heat_fixture(lbm), "voxelize_small.npz": vox_fixture()}
+    lines = []
+    for name, data in files.items():
+        np.savez_compressed(HERE / name, **data)
+        lines.append(f"{hashlib.sha256((HERE / name).read_bytes()).hexdigest()}  {name}")
+    (HERE / "MANIFEST.sha256").write_text("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+if __name__ == "__main__":
+    main()
