@@ -81,6 +81,7 @@ _PROTOS = {
     "aero_init": (C.c_int, [_P]),
     "aero_voxelize": (C.c_int, [_P, C.POINTER(Shape), _P, _P, _U32]),
     "aero_set_car_model": (C.c_int, [_P, _P, _P, _P, C.POINTER(Rig)]),
+    "aero_set_rig": (C.c_int, [_P, C.POINTER(Rig)]),
     "aero_set_flags": (C.c_int, [_P, _P, _P]),
     "aero_step": (C.c_int, [_P, _U32]),
     "aero_step_timed": (C.c_int, [_P, _U32, C.POINTER(C.c_float)]),

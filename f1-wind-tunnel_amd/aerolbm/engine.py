@@ -66,6 +66,13 @@ class ShapeParams:
     heatEngine: bool = True
     heatRad: bool = True
     cooling: int = 1
+    # W17 rig inputs (engine.ts:451-462)
+    rearShape: int = 0
+    thinWings: bool = False
+    sidepods: int = 0
+    frontWingOn: bool = True
+    rearWingOn: bool = True
+    gurney: bool = False
 
 
 @dataclass
@@ -217,8 +224,28 @@ class HipAeroEngine:
     def set_profile(self, profile: CustomProfile) -> None:
         self.profile = profile
 
+    def set_car_model(self, body: np.ndarray, parts: np.ndarray, rig: dict) -> None:
+        """setCarModel (engine.ts:410-422): the W17 body grid (368x84x67 f16 words),
+        the concatenated part grids and the rig description (w17.rig.json)."""
+        from . import w17
+
+        dims, atlas = w17.parts_atlas(rig, np.asarray(parts, np.uint16))
+        b = np.ascontiguousarray(body, dtype=np.uint16)
+        d = np.array(dims, np.uint32)
+        self.car_rig = rig
+        r = w17.rig_uniform(rig, [0.0, 0.0, 0.0, 0.0])
+        self._c(self._lib.aero_set_car_model(self.ctx, ptr(b), ptr(atlas), ptr(d), C.byref(r)), "aero_set_car_model")
+
+    def set_rig(self, rig) -> None:
+        self._c(self._lib.aero_set_rig(self.ctx, C.byref(rig)), "aero_set_rig")
+
     def set_shape(self, p, prims: Optional[Sequence[Prim]] = None) -> None:
-        """engine.ts:426-470: packs the Shape uniform and voxelizes."""
+        """engine.ts:426-470: packs the Shape uniform (and, for the W17, the rig it
+        derives from the setup) and voxelizes."""
+        if not isinstance(p, Shape) and p.kind == ShapeKind.W17 and getattr(self, "car_rig", None) is not None:
+            from . import w17
+
+            self.set_rig(w17.rig_for_setup(self.car_rig, p))
         shape = p if isinstance(p, Shape) else shape_uniform(p, self.profile)
         prof = None
         if shape.kind not in (ShapeKind.Sphere, ShapeKind.Cube, ShapeKind.Wing, ShapeKind.Empty,

@@ -119,6 +119,7 @@ struct aero_ctx {
     uint16_t* car_parts;
     uint32_t parts_dims[3];
     aero_rig rig;
+    aero_rig* d_rig;  // device copy read by the voxelizer
     int has_car;
     // Live timing of stream-collide bursts (aero_profile_begin/end): one event pair
     // per aero_step call, recorded on the context stream.

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "aero_internal.h"
+#include "sdf_w17.h"
 #include "sdf_f1.h"
 
 using namespace aero;
@@ -73,7 +74,7 @@ void free_ctx(aero_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     void* ptrs[] = {c->f[0], c->f[1], c->solid, c->emit, c->flags, c->fields, c->prev_u, c->temps[0],
-                    c->temps[1], c->tau_x, c->slots, c->red_out, c->car_body, c->car_parts};
+                    c->temps[1], c->tau_x, c->slots, c->red_out, c->car_body, c->car_parts, c->d_rig};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t ev : c->prof_ev) (void)hipEventDestroy(ev);
@@ -282,9 +283,60 @@ int aero_voxelize(aero_ctx* c, const aero_shape* s, const float* profile, const 
     return rc;
 }
 
-int aero_set_car_model(aero_ctx* c, const uint16_t*, const uint16_t*, const uint32_t*, const aero_rig*) {
+int aero_set_car_model(aero_ctx* c, const uint16_t* body_sdf, const uint16_t* parts_atlas, const uint32_t atlas_dims[3],
+                       const aero_rig* rig) {
     CTXCHK(c);
-    return fail(c, AERO_E_INVALID, "aero_set_car_model: W17 model not available in this build");
+    if (!body_sdf || !parts_atlas || !atlas_dims || !rig)
+        return fail(c, AERO_E_INVALID, "aero_set_car_model: null argument");
+    const uint64_t na = (uint64_t)atlas_dims[0] * atlas_dims[1] * atlas_dims[2];
+    if (na == 0 || atlas_dims[0] > 4096 || atlas_dims[1] > 4096 || atlas_dims[2] > 4096 || na > (1ull << 28))
+        return fail(c, AERO_E_INVALID, "aero_set_car_model: atlas dims out of range");
+    // every part grid of the rig must lie inside the atlas (car-model.ts:174-190)
+    for (int part = 0; part < 6; part++) {
+        const float* g1 = rig->grids[part * 2 + 1];
+        if (g1[0] < 1.0f || g1[1] < 1.0f || g1[2] < 1.0f || g1[0] > (float)atlas_dims[0] ||
+            g1[1] > (float)atlas_dims[1] || g1[3] < 0.0f || g1[3] + g1[2] > (float)atlas_dims[2] ||
+            !(rig->grids[part * 2][3] > 0.0f))
+            return fail(c, AERO_E_INVALID, "aero_set_car_model: rig part grid outside the atlas");
+    }
+    const size_t nb = (size_t)w17::kNx * w17::kNy * w17::kNz;
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->car_body) (void)hipFree(c->car_body);
+    if (c->car_parts) (void)hipFree(c->car_parts);
+    c->car_body = nullptr;
+    c->car_parts = nullptr;
+    c->has_car = 0;
+    HIPCHK(c, hipMalloc(&c->car_body, nb * 2));
+    HIPCHK(c, hipMalloc(&c->car_parts, na * 2));
+    if (!c->d_rig) HIPCHK(c, hipMalloc(&c->d_rig, sizeof(aero_rig)));
+    HIPCHK(c, hipMemcpy(c->car_body, body_sdf, nb * 2, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->car_parts, parts_atlas, na * 2, hipMemcpyHostToDevice));
+    HIPCHK(c, hipMemcpy(c->d_rig, rig, sizeof(aero_rig), hipMemcpyHostToDevice));
+    c->rig = *rig;
+    c->parts_dims[0] = atlas_dims[0];
+    c->parts_dims[1] = atlas_dims[1];
+    c->parts_dims[2] = atlas_dims[2];
+    c->has_car = 1;
+    return AERO_OK;
+}
+
+int aero_set_rig(aero_ctx* c, const aero_rig* rig) {
+    CTXCHK(c);
+    if (!rig) return fail(c, AERO_E_INVALID, "aero_set_rig: null rig");
+    if (!c->has_car) return fail(c, AERO_E_STATE, "aero_set_rig: no car model (aero_set_car_model)");
+    for (int part = 0; part < 6; part++) {
+        const float* g1 = rig->grids[part * 2 + 1];
+        if (g1[0] < 1.0f || g1[1] < 1.0f || g1[2] < 1.0f || g1[0] > (float)c->parts_dims[0] ||
+            g1[1] > (float)c->parts_dims[1] || g1[3] < 0.0f || g1[3] + g1[2] > (float)c->parts_dims[2] ||
+            !(rig->grids[part * 2][3] > 0.0f))
+            return fail(c, AERO_E_INVALID, "aero_set_rig: rig part grid outside the atlas");
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_rig, rig, sizeof(aero_rig), hipMemcpyHostToDevice));
+    c->rig = *rig;
+    return AERO_OK;
 }
 
 int aero_set_flags(aero_ctx* c, const uint32_t* solid, const float* emit) {

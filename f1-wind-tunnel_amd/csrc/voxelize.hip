@@ -6,6 +6,7 @@
 #include "aero_internal.h"
 #include "sdf.h"
 #include "sdf_f1.h"
+#include "sdf_w17.h"
 
 namespace aero {
 
@@ -21,6 +22,10 @@ struct VoxArgs {
     float rake_c, rake_s;   // cos/sin of the rake angle (f1.ts:37)
     float wing_c, wing_s;   // cos/sin(-aoa) for kind 3
     float rw_cos, rw_sin;   // cos/sin of the rear main-plane angle (kind 0)
+    const uint16_t* car_body;   // W17 (kind 5): body grid, parts atlas, rig
+    const uint16_t* car_parts;
+    uint32_t atlas[3];
+    const aero_rig* rig;
     uint32_t* solid;
     float* emit;
 };
@@ -89,6 +94,10 @@ __device__ Res scene_sdf(const VoxArgs& a, v3 pw, float* emission) {
     return r;
 }
 
+__device__ __forceinline__ w17::Ctx w17_ctx(const VoxArgs& a) {
+    return w17::Ctx{&a.shape, a.rig, a.car_body, a.car_parts, a.atlas[0], a.atlas[1], a.atlas[2], a.rake_c, a.rake_s};
+}
+
 // sceneSdf (wgsl.ts:101-111) and sceneEmission (wgsl.ts:112-116).
 __device__ Res scene(const VoxArgs& a, v3 p, float* emission) {
     *emission = 0.0f;
@@ -105,6 +114,12 @@ __device__ Res scene(const VoxArgs& a, v3 p, float* emission) {
         return Res{sdWingCS(p, mk2(1.6f, 1.2f), 1.8f, a.wing_c, a.wing_s, 0.24f, -1.25f, 1.25f), 6.0f};
     case AERO_KIND_EMPTY: return Res{1e3f, 0.0f};
     case AERO_KIND_SCENE: return scene_sdf(a, p, emission);
+    case AERO_KIND_W17: {
+        const w17::Ctx c = w17_ctx(a);
+        const w17::R2 r = w17::sdW17(c, p);
+        *emission = w17::w17Emission(c, p);
+        return Res{r.d, r.m};
+    }
     default: return Res{sd_custom(a, p), 6.0f};
     }
 }
@@ -117,13 +132,21 @@ __global__ void __launch_bounds__(256) k_voxelize(VoxArgs a) {
     const uint64_t idx = (uint64_t)cx + (uint64_t)a.nx * ((uint64_t)cy + (uint64_t)a.ny * cz);
     const v3 p = mk3(a.ox + ((float)cx + 0.5f) * a.h, a.oy + ((float)cy + 0.5f) * a.h, a.oz + ((float)cz + 0.5f) * a.h);
     float em = 0.0f;
-    const Res r = scene(a, p, &em);
-    const bool inside = r.d < 0.3f * a.h && cx > 1u && cx < a.nx - 2u;
+    Res r = scene(a, p, &em);
+    // W17: the sidepod cooling ducts are carved out of the body (material 12) and
+    // the car is voxelized with a wider band (wgsl.ts:313-314,319-320).
+    const bool is_w17 = a.shape.kind == AERO_KIND_W17;
+    float duct = 1e3f;
+    if (is_w17) {
+        duct = w17::w17Duct(w17_ctx(a), p);
+        if (fabs_(r.m - 12.0f) < 0.5f) r.d = fmax_(r.d, -duct);
+    }
+    const bool inside = r.d < (is_w17 ? 0.45f : 0.3f) * a.h && cx > 1u && cx < a.nx - 2u;
     uint32_t group = 1u;
     if (fabs_(r.m - 7.0f) < 0.5f) group = 2u;
     if (fabs_(r.m - 8.0f) < 0.5f) group = 3u;
     a.solid[idx] = inside ? group : 0u;
-    a.emit[idx] = inside ? em : 0.0f;
+    a.emit[idx] = inside ? em : (duct < 0.0f ? -1.0f : 0.0f);
 }
 
 hipError_t launch_voxelize(aero_ctx* c, const aero_shape* s, const float* d_profile, const aero_prim* d_prims,
@@ -138,6 +161,10 @@ hipError_t launch_voxelize(aero_ctx* c, const aero_shape* s, const float* d_prof
     a.rake_c = host_extra[0]; a.rake_s = host_extra[1];
     a.wing_c = host_extra[2]; a.wing_s = host_extra[3];
     a.rw_cos = host_extra[4]; a.rw_sin = host_extra[5];
+    a.car_body = c->car_body;
+    a.car_parts = c->car_parts;
+    a.atlas[0] = c->parts_dims[0]; a.atlas[1] = c->parts_dims[1]; a.atlas[2] = c->parts_dims[2];
+    a.rig = c->d_rig;
     a.solid = c->solid;
     a.emit = c->emit;
     const dim3 grid((c->grid.nx + 63) / 64, (c->grid.ny + 3) / 4, c->grid.nz);

@@ -128,6 +128,10 @@ int aero_voxelize(aero_ctx* ctx, const aero_shape* shape, const float* profile,
  * (car-model.ts:174-190) with dims atlas_dims. */
 int aero_set_car_model(aero_ctx* ctx, const uint16_t* body_sdf, const uint16_t* parts_atlas,
                        const uint32_t atlas_dims[3], const aero_rig* rig);
+/* The rig uniform alone (engine.ts:451-462: setShape recomputes it from the wing
+ * angles, rear-wing shape, flap scale, sidepod squeeze and fitted wings, then
+ * voxelizes).  Needs aero_set_car_model first. */
+int aero_set_rig(aero_ctx* ctx, const aero_rig* rig);
 
 /* Test/driver hook: replaces solid (u32 groups 0..3) and emit (f32) directly. */
 int aero_set_flags(aero_ctx* ctx, const uint32_t* solid, const float* emit);

@@ -149,10 +149,23 @@ class SDFOracle:
         L.orc_sdf_point.restype = C.c_float
         L.orc_sdf_point.argtypes = [C.POINTER(OShape), C.c_void_p, C.c_void_p, C.c_uint32, C.c_float, C.c_float,
                                     C.c_float, C.POINTER(C.c_float)]
+        L.orc_set_car.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_half_to_float.restype = C.c_float
+        L.orc_half_to_float.argtypes = [C.c_uint16]
+        self._car = None
         for name, nargs in (("orc_sd_wing", 10), ("orc_sd_wheel", 8), ("orc_sd_roundbox", 7)):
             fn = getattr(L, name)
             fn.restype = C.c_float
             fn.argtypes = [C.c_float] * nargs
+
+    def set_car(self, body: np.ndarray, atlas: np.ndarray, atlas_dims, rig) -> None:
+        """W17 grids (u16 f16 words) and the Rig struct (any ctypes struct with the
+        aero_rig layout); the arrays are kept alive by this object."""
+        b = np.ascontiguousarray(body, dtype=np.uint16)
+        a = np.ascontiguousarray(atlas, dtype=np.uint16)
+        d = np.ascontiguousarray(atlas_dims, dtype=np.uint32)
+        self._car = (b, a, d, rig)
+        self.lib.orc_set_car(_p(b), _p(a), _p(d), C.cast(C.byref(rig), C.c_void_p))
 
     def voxelize(self, g: OGrid, shape, profile=None, prims=None):
         sh = as_shape(shape)

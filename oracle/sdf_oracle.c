@@ -12,6 +12,11 @@
  *
  *   src/engine/f1.ts:20-320     analytic F1 car (sdF1 and its parts), kind 0
  *   src/engine/f1.ts:397-407    f1Emission
+ *   src/engine/f1.ts:322-395    W17 baked car (body grid, articulated parts, tyres,
+ *                               cooling ducts, emission), kind 5, with the rig of
+ *                               src/engine/car-model.ts:59-136; the r16float textures
+ *                               are sampled by an explicit FP32 trilinear filter with
+ *                               clamp-to-edge addressing (engine.ts:271)
  *
  * Kind 16 ("primitive scene") is this build's own extension: a union of the
  * reference's primitives with per-primitive material ids, evaluated in the world
@@ -523,12 +528,211 @@ static float scene_emission(const og_shape* sh, const og_prim* prims, uint32_t n
     return e;
 }
 
-/* sceneSdf, wgsl.ts:101-111 (kinds 0 and 5 are not restated here) */
+
+/* ---- W17 baked car, kind 5: f1.ts:322-395 with the rig of car-model.ts:59-136 ---- */
+typedef struct {
+    float front[71][4];
+    float hinges[4], angles[4], extra[4], fly[4];
+    float grids[12][4];
+} og_rig; /* struct Rig, car-model.ts:60-69; layout shared with aero_rig */
+
+static const uint16_t* g_body;  /* 368 x 84 x 67 f16, x fastest (engine.ts:412) */
+static const uint16_t* g_parts; /* partsAtlas packing, car-model.ts:174-190 */
+static uint32_t g_atlas[3];
+static og_rig g_rig;
+
+#define W17_OX (-0.3f)
+#define W17_H 0.015f
+#define W17_NX 368
+#define W17_NY 84
+#define W17_NZ 67
+#define W17_BASE_RIDE 0.092f
+
+/* IEEE binary16 -> binary32 (exact) */
+static float half_to_float(uint16_t h) {
+    uint32_t sign = (uint32_t)(h >> 15) << 31, ex = (h >> 10) & 0x1fu, man = h & 0x3ffu, bits;
+    if (ex == 0u) {
+        if (man == 0u) bits = sign;
+        else { /* subnormal: man * 2^-24 */
+            float f = (float)man * 5.9604644775390625e-8f;
+            return sign ? -f : f;
+        }
+    } else if (ex == 31u) {
+        bits = sign | 0x7f800000u | (man << 13);
+    } else {
+        bits = sign | ((ex + 112u) << 23) | (man << 13);
+    }
+    float f;
+    memcpy(&f, &bits, 4);
+    return f;
+}
+
+/* Trilinear filtering with clamp-to-edge addressing at continuous texel coordinates
+ * (texel centres at integers): the sampler of engine.ts:271, evaluated in FP32. */
+static float tex_at(const uint16_t* t, uint32_t nx, uint32_t ny, uint32_t nz, int x, int y, int z) {
+    x = x < 0 ? 0 : (x > (int)nx - 1 ? (int)nx - 1 : x);
+    y = y < 0 ? 0 : (y > (int)ny - 1 ? (int)ny - 1 : y);
+    z = z < 0 ? 0 : (z > (int)nz - 1 ? (int)nz - 1 : z);
+    return half_to_float(t[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * (size_t)z)]);
+}
+static float tex3(const uint16_t* t, uint32_t nx, uint32_t ny, uint32_t nz, float u, float v, float w) {
+    float fu = floorf(u), fv = floorf(v), fw = floorf(w);
+    int x0 = (int)fu, y0 = (int)fv, z0 = (int)fw;
+    float a = u - fu, b = v - fv, c = w - fw;
+    float c00 = mixf(tex_at(t, nx, ny, nz, x0, y0, z0), tex_at(t, nx, ny, nz, x0 + 1, y0, z0), a);
+    float c10 = mixf(tex_at(t, nx, ny, nz, x0, y0 + 1, z0), tex_at(t, nx, ny, nz, x0 + 1, y0 + 1, z0), a);
+    float c01 = mixf(tex_at(t, nx, ny, nz, x0, y0, z0 + 1), tex_at(t, nx, ny, nz, x0 + 1, y0, z0 + 1), a);
+    float c11 = mixf(tex_at(t, nx, ny, nz, x0, y0 + 1, z0 + 1), tex_at(t, nx, ny, nz, x0 + 1, y0 + 1, z0 + 1), a);
+    return mixf(mixf(c00, c10, b), mixf(c01, c11, b), c);
+}
+
+/* car-model.ts:71-113 */
+static float rigWeight(uint32_t part, float z) {
+    if (part == 1u || part == 2u) {
+        float az = fabsf(z);
+        return smoothstepf(0.16f, 0.22f, az) * (1.0f - smoothstepf(0.5f, 0.56f, az));
+    }
+    return (part == 3u || part == 4u) ? 1.0f : 0.0f;
+}
+static f2 rigHinge(uint32_t part, float z) {
+    if (part == 3u) return v2(g_rig.hinges[2], g_rig.hinges[3]);
+    if (part == 4u) return v2(g_rig.hinges[0], g_rig.hinges[1]);
+    float f = clampf_(fabsf(z) / 0.01f, 0.0f, 70.0f);
+    uint32_t i = (uint32_t)floorf(f);
+    uint32_t j = i + 1u < 70u ? i + 1u : 70u;
+    const float* a = g_rig.front[i];
+    const float* b = g_rig.front[j];
+    float t = f - (float)i;
+    if (part == 1u) return v2(mixf(a[0], b[0], t), mixf(a[1], b[1], t));
+    return v2(mixf(a[2], b[2], t), mixf(a[3], b[3], t));
+}
+static float rigAngle(uint32_t part, float z) {
+    float a = 0.0f;
+    if (part == 1u) a = g_rig.angles[0];
+    else if (part == 2u) a = g_rig.angles[1];
+    else if (part == 3u) a = g_rig.angles[2];
+    else if (part == 4u) a = g_rig.angles[3];
+    if (part == 3u || part == 4u) a += g_rig.extra[0] * smoothstepf(0.12f, 0.48f, fabsf(z));
+    return a * rigWeight(part, z);
+}
+static float rigScale(uint32_t part) {
+    if (part == 3u) return g_rig.extra[1];
+    if (part == 1u || part == 2u) return 1.0f - (1.0f - g_rig.extra[1]) * 0.65f;
+    return 1.0f;
+}
+static f3 rigRotateInv(f3 p, uint32_t part) { /* rigRotate(p, part, -1) */
+    if (part == 0u || part > 4u) return p;
+    float a = -rigAngle(part, p.z);
+    float k = rigScale(part);
+    f2 h = rigHinge(part, p.z);
+    float c = cosf(a), s = sinf(a);
+    f2 d = sub2(v2(p.x, p.y), h);
+    d = v2(c * d.x - s * d.y, s * d.x + c * d.y);
+    d = v2(d.x / k, d.y / k);
+    return v3(h.x + d.x, h.y + d.y, p.z);
+}
+/* car-model.ts:122-135 */
+static float podWeight(f3 p) {
+    return smoothstepf(1.85f, 2.25f, p.x) * (1.0f - smoothstepf(3.55f, 4.05f, p.x)) * smoothstepf(0.12f, 0.2f, p.y) *
+           (1.0f - smoothstepf(0.6f, 0.7f, p.y)) * smoothstepf(0.26f, 0.4f, fabsf(p.z));
+}
+static f3 podDeformInv(f3 p) { /* podDeform(p, -1) */
+    float c = g_rig.extra[2] * podWeight(p);
+    if (c <= 0.0f) return p;
+    float az = fabsf(p.z);
+    float kz = 1.0f / (1.0f - c);
+    float ky = 1.0f / (1.0f - 0.45f * c);
+    float nz = az > 0.26f ? 0.26f + (az - 0.26f) * kz : az;
+    float sg = p.z > 0.0f ? 1.0f : (p.z < 0.0f ? -1.0f : 0.0f);
+    return v3(p.x, 0.12f + (p.y - 0.12f) * ky, sg * nz);
+}
+
+/* f1.ts:330-381 */
+static f3 w17BodyFrame(f3 pw) {
+    float th = rakeAngle();
+    float c = cosf(th), s = sinf(th);
+    float dx = pw.x - FA, dy = pw.y - g_sh->rideH;
+    return v3(FA + dx * c + dy * s, W17_BASE_RIDE - dx * s + dy * c, pw.z);
+}
+static float sdW17Body(f3 p) {
+    f3 ext = v3(W17_NX * W17_H, W17_NY * W17_H, W17_NZ * W17_H);
+    f3 half = v3(ext.x * 0.5f, ext.y * 0.5f, ext.z * 0.5f);
+    float box = sdBox(sub3(p, v3(W17_OX + half.x, half.y, half.z)), half);
+    if (box > 0.0f) return box + 0.07f;
+    return tex3(g_body, W17_NX, W17_NY, W17_NZ, (p.x - W17_OX) / W17_H, p.y / W17_H, p.z / W17_H);
+}
+static float sdW17Part(f3 pb, uint32_t part) {
+    const float* g0 = g_rig.grids[(part - 1u) * 2u];
+    const float* g1 = g_rig.grids[(part - 1u) * 2u + 1u];
+    f3 q = rigRotateInv(pb, part);
+    f3 ext = v3((g1[0] - 1.0f) * g0[3], (g1[1] - 1.0f) * g0[3], (g1[2] - 1.0f) * g0[3]);
+    f3 half = v3(ext.x * 0.5f, ext.y * 0.5f, ext.z * 0.5f);
+    float box = sdBox(sub3(q, v3(g0[0] + half.x, g0[1] + half.y, g0[2] + half.z)), half);
+    if (box > 0.0f) return box + 0.03f;
+    f3 cell = v3((q.x - g0[0]) / g0[3], (q.y - g0[1]) / g0[3], (q.z - g0[2]) / g0[3]);
+    return tex3(g_parts, g_atlas[0], g_atlas[1], g_atlas[2], cell.x, cell.y, cell.z + g1[3]) *
+           mn(rigScale(part), 1.0f);
+}
+static res2 sdW17(f3 pw) {
+    static const f3 FRONT = {1.0f, 0.36f, 0.779f}, REAR = {4.4f, 0.351f, 0.737f};
+    f3 pa = v3(pw.x, pw.y, fabsf(pw.z));
+    f3 p = w17BodyFrame(pa);
+    float squeeze = 1.0f - g_rig.extra[2] * podWeight(p);
+    res2 res = {sdW17Body(podDeformInv(p)) * 0.95f * squeeze, 12.0f};
+    if (g_rig.fly[2] > 0.5f && p.x < 0.85f && p.y < 0.6f) {
+        res2 c = {mn(mn(sdW17Part(p, 1u), sdW17Part(p, 2u)), sdW17Part(p, 5u)) * 0.95f, 7.0f};
+        res = opU(res, c);
+    }
+    if (g_rig.fly[3] > 0.5f && p.x > 4.4f && p.y > 0.5f) {
+        res2 c = {mn(mn(sdW17Part(p, 3u), sdW17Part(p, 4u)), sdW17Part(p, 6u)) * 0.95f, 8.0f};
+        res = opU(res, c);
+    }
+    res2 tf = {mn(sdTyre(pa, FRONT, 0.175f), sdCylZ(sub3(pa, FRONT), 0.234f, 0.15f)), 2.0f};
+    res = opU(res, tf);
+    res2 tr = {mn(sdTyre(pa, REAR, 0.205f), sdCylZ(sub3(pa, REAR), 0.234f, 0.18f)), 2.0f};
+    return opU(res, tr);
+}
+static float w17Duct(f3 pw) {
+    if (g_sh->duct <= 0.0f) return 1e3f;
+    f3 p = podDeformInv(w17BodyFrame(v3(pw.x, pw.y, fabsf(pw.z))));
+    float k = g_sh->duct;
+    float mouth = sdCapsule(p, v3(2.05f, 0.52f, 0.58f), v3(3.0f, 0.50f, 0.48f), 0.09f * k);
+    float mainc = sdCapsule(p, v3(2.05f, 0.55f, 0.47f), v3(3.25f, 0.46f, 0.43f), 0.1f * k);
+    float exitc = sdCapsule(p, v3(3.25f, 0.46f, 0.43f), v3(4.1f, 0.40f, 0.32f), 0.085f * k);
+    return mn(mn(mouth, mainc), exitc);
+}
+static float w17Emission(f3 pw) {
+    static const f3 FRONT = {1.0f, 0.36f, 0.779f}, REAR = {4.4f, 0.351f, 0.737f};
+    f3 pa = v3(pw.x, pw.y, fabsf(pw.z));
+    f3 p = w17BodyFrame(pa);
+    float e = 0.0f;
+    if (len2(v2(pa.x - FRONT.x, pa.y - FRONT.y)) < 0.2f && fabsf(pa.z - (FRONT.z - 0.215f)) < 0.1f) e = mx(e, g_sh->heatBrakes);
+    if (len2(v2(pa.x - REAR.x, pa.y - REAR.y)) < 0.2f && fabsf(pa.z - (REAR.z - 0.245f)) < 0.1f) e = mx(e, g_sh->heatBrakes * 0.8f);
+    if (len3(sub3(p, v3(5.07f, 0.53f, 0.0f))) < 0.1f) e = mx(e, g_sh->heatEngine);
+    if (p.x > 3.7f && p.x < 4.3f && p.y > 0.2f && p.y < 0.5f && p.z > 0.18f && p.z < 0.48f) e = mx(e, g_sh->heatRad);
+    if (p.x > 4.15f && p.x < 4.65f && p.y > 0.5f && p.z < 0.16f) e = mx(e, g_sh->heatRad * 1.4f);
+    return e;
+}
+
+/* setCarModel (engine.ts:410-422) for the oracle: the arrays are borrowed until the
+ * next call. */
+EXPORT void orc_set_car(const uint16_t* body, const uint16_t* parts, const uint32_t* atlas_dims, const og_rig* rig) {
+    g_body = body;
+    g_parts = parts;
+    g_atlas[0] = atlas_dims[0];
+    g_atlas[1] = atlas_dims[1];
+    g_atlas[2] = atlas_dims[2];
+    g_rig = *rig;
+}
+EXPORT float orc_half_to_float(uint16_t h) { return half_to_float(h); }
+
+/* sceneSdf, wgsl.ts:101-111 */
 static res2 sceneSdf(const og_shape* sh, const float* prof, const og_prim* prims, uint32_t np, f3 p) {
     res2 r;
     g_sh = sh;
     switch (sh->kind) {
     case 0u: return sdF1(p);
+    case 5u: return sdW17(p);
     case 1u: r.d = len3(sub3(p, v3(2.6f, 1.0f, 0.0f))) - 0.6f; r.m = 6.0f; return r;
     case 2u: r.d = sdRoundBox(sub3(p, v3(2.6f, 1.0f, 0.0f)), v3(0.5f, 0.5f, 0.5f), 0.03f); r.m = 6.0f; return r;
     case 3u: r.d = sdWing(p, v2(1.6f, 1.2f), 1.8f, -sh->aoa, 0.24f, -1.25f, 1.25f); r.m = 6.0f; return r;
@@ -540,6 +744,7 @@ static res2 sceneSdf(const og_shape* sh, const float* prof, const og_prim* prims
 static float sceneEmission(const og_shape* sh, const og_prim* prims, uint32_t np, f3 p) {
     g_sh = sh;
     if (sh->kind == 0u) return f1Emission(p);
+    if (sh->kind == 5u) return w17Emission(p);
     if (sh->kind == 16u) return scene_emission(sh, prims, np, p);
     return 0.0f;
 }
@@ -555,12 +760,15 @@ EXPORT void orc_voxelize(const og_grid* g, const og_shape* sh, const float* prof
         f3 p = v3(g->ox + ((float)cx + 0.5f) * g->h, g->oy + ((float)cy + 0.5f) * g->h,
                   g->oz + ((float)cz + 0.5f) * g->h);
         res2 r = sceneSdf(sh, profile, prims, nprims, p);
-        int inside = r.d < 0.3f * g->h && cx > 1u && cx < NX - 2u;
+        const int w17 = sh->kind == 5u;
+        if (w17 && fabsf(r.m - 12.0f) < 0.5f) r.d = mx(r.d, -w17Duct(p));
+        int inside = r.d < (w17 ? 0.45f : 0.3f) * g->h && cx > 1u && cx < NX - 2u;
         uint32_t group = 1u;
         if (fabsf(r.m - 7.0f) < 0.5f) group = 2u;
         if (fabsf(r.m - 8.0f) < 0.5f) group = 3u;
         solid[idx] = inside ? group : 0u;
-        emit[idx] = inside ? sceneEmission(sh, prims, nprims, p) : 0.0f;
+        const int duct = w17 && w17Duct(p) < 0.0f;
+        emit[idx] = inside ? sceneEmission(sh, prims, nprims, p) : (duct ? -1.0f : 0.0f);
         if (sdf_out) sdf_out[idx] = r.d;
     }
 }

@@ -107,13 +107,34 @@ def vox_fixture():
     return out
 
 
+REF_MODELS = Path("/root/reference/public/models")
+
+
+def w17_rig_fixture() -> None:
+    """The articulation data of the reference's W17 model (public/models/w17.rig.json,
+    a data file): its hinges and part-grid placements, without the render-only wing
+    stations and sections.  Written only where the reference is present; the committed
+    copy serves everywhere else."""
+    src = REF_MODELS / "w17.rig.json"
+    if not src.exists():
+        return
+    import json
+
+    rig = json.loads(src.read_text())
+    keep = {k: rig[k] for k in ("frontHinges", "rearMainHinge", "rearFlapHinge", "partGrids")}
+    (HERE / "w17_rig.json").write_text(json.dumps(keep, separators=(",", ":")) + "\n")
+
+
 def main() -> None:
     lbm = lbm_fixture()
     files = {"lbm_mixed_24x12x8.npz": lbm, "heat_24x12x8.npz": heat_fixture(lbm), "voxelize_small.npz": vox_fixture()}
+    w17_rig_fixture()
     lines = []
     for name, data in files.items():
         np.savez_compressed(HERE / name, **data)
         lines.append(f"{hashlib.sha256((HERE / name).read_bytes()).hexdigest()}  {name}")
+    name = "w17_rig.json"
+    lines.append(f"{hashlib.sha256((HERE / name).read_bytes()).hexdigest()}  {name}")
     (HERE / "MANIFEST.sha256").write_text("\n".join(lines) + "\n")
     print("\n".join(lines))
 

@@ -210,3 +210,98 @@ def test_f1_rake_lifts_the_rear():
     assert py[(raked != 0) & rear].mean() > py[(flat != 0) & rear].mean() + 0.02
     front = (px > 0.2) & (px < 1.5)
     assert abs(py[(raked != 0) & front].mean() - py[(flat != 0) & front].mean()) < 0.03
+
+
+# ---- W17 baked car (kind 5) ----
+import json
+from pathlib import Path
+
+from aerolbm import w17 as W17
+from aerolbm._lib import Rig
+
+RIG = json.loads((Path(__file__).resolve().parent / "golden" / "w17_rig.json").read_text())
+REF_MODELS = Path("/root/reference/public/models")
+
+
+def test_half_decoder_is_exact_for_every_word():
+    words = np.arange(65536, dtype=np.uint32)
+    got = np.array([SDF.lib.orc_half_to_float(int(w)) for w in words], np.float32)
+    want = words.astype(np.uint16).view(np.float16).astype(np.float32)
+    nan = np.isnan(want)
+    assert np.array_equal(np.isnan(got), nan)
+    assert np.array_equal(got[~nan].view(np.uint32), want[~nan].view(np.uint32))
+
+
+def test_parts_atlas_packing():
+    body, parts = W17.synthetic_car(RIG)
+    dims, atlas = W17.parts_atlas(RIG, parts)
+    grids = RIG["partGrids"]
+    assert dims == (max(g["dims"][0] for g in grids), max(g["dims"][1] for g in grids),
+                    sum(g["dims"][2] + 1 for g in grids))
+    a = atlas.reshape(dims[2], dims[1], dims[0])
+    z0 = 0
+    for g in grids:
+        gx, gy, gz = g["dims"]
+        src = parts[g["offset"]:g["offset"] + gx * gy * gz].reshape(gz, gy, gx)
+        assert np.array_equal(a[z0:z0 + gz, :gy, :gx], src)
+        assert (a[z0 + gz] == W17.ATLAS_FAR).all()        # padding slice
+        assert (a[z0:z0 + gz, :, gx:] == W17.ATLAS_FAR).all()
+        z0 += gz + 1
+
+
+def w17_shape(ride=0.092, rear=0.092, duct=1.0):
+    s = shape(5, rideH=ride, rearRide=rear)
+    s.heatBrakes, s.heatEngine, s.heatRad, s.duct = 520.0, 620.0, 75.0, duct
+    return s
+
+
+def test_w17_body_is_the_trilinear_grid():
+    """With no rake (ride = base ride 0.092 front and rear) and no pod squeeze the
+    body frame is the identity, and away from wheels and parts the W17 distance is
+    0.95 x the trilinear interpolation of the body grid (f1.ts:334-340,357)."""
+    body, parts = W17.synthetic_car(RIG)
+    dims, atlas = W17.parts_atlas(RIG, parts)
+    SDF.set_car(body, atlas, dims, W17.rig_uniform(RIG, [0, 0, 0, 0]))
+    grid = body.view(np.float16).astype(np.float64).reshape(67, 84, 368)
+    rng = np.random.default_rng(9)
+    pts = np.column_stack([rng.uniform(1.6, 3.4, 300), rng.uniform(0.8, 1.15, 300), rng.uniform(-0.9, 0.9, 300)])
+    s = w17_shape(duct=0.0)
+    checked = 0
+    for x, y, z in pts:
+        u, v, w = (x + 0.3) / 0.015, y / 0.015, abs(z) / 0.015
+        i, j, k = int(np.floor(u)), int(np.floor(v)), int(np.floor(w))
+        a, b, c = u - i, v - j, w - k
+        cl = lambda t, n: min(max(t, 0), n - 1)
+        val = 0.0
+        for di, wa in ((0, 1 - a), (1, a)):
+            for dj, wb in ((0, 1 - b), (1, b)):
+                for dk, wc in ((0, 1 - c), (1, c)):
+                    val += wa * wb * wc * grid[cl(k + dk, 67), cl(j + dj, 84), cl(i + di, 368)]
+        d, m = SDF.sdf_point(s, float(x), float(y), float(z))
+        if m == 12.0:
+            assert abs(d - 0.95 * val) < 2e-5
+            checked += 1
+    assert checked > 250
+
+
+@pytest.mark.skipif(not REF_MODELS.exists(), reason="the reference's model files are not present")
+def test_w17_reference_model_oracle():
+    """The shipped W17 grids at the app's medium tunnel: the car sits between the
+    wheel axles' extremes, front- and rear-wing groups come from the articulated parts
+    at the car's ends, the sidepod ducts are carved (emit = -1 in fluid duct cells)
+    and brake heat sits at both axles."""
+    body, parts, rig = W17.load_reference_car(REF_MODELS)
+    dims, atlas = W17.parts_atlas(rig, parts)
+    SDF.set_car(body, atlas, dims, W17.rig_uniform(rig, [0, 0, 0, 0]))
+    h = 10.4 / 232
+    g = orc.grid((232, 68, 52), h, (-2.4, 0.0, 0.0))
+    solid, emit, _ = SDF.voxelize(g, w17_shape(ride=0.12, rear=0.14))
+    x, px, py, pz = centres(g)
+    m = solid != 0
+    assert m.sum() > 10000
+    assert px[m].min() > -0.4 and px[m].max() < 5.4 and py[m].max() < 1.3 and pz[m].max() < 1.05
+    fw, rw = solid == 2, solid == 3
+    assert fw.sum() > 50 and rw.sum() > 50
+    assert px[fw].max() < 0.9 and px[rw].min() > 4.3
+    assert (emit == -1.0).sum() > 20 and ((emit == -1.0) & m).sum() == 0
+    assert np.isin([520.0, 416.0], np.unique(emit[m])).all()
