@@ -60,13 +60,14 @@ struct StepArgs {
     const void* fin;
     void* fout;
     const uint8_t* flags;
+    const uint32_t* fastmap;  // bit per 64-cell row segment: every cell plain interior fluid
+    uint32_t nseg;            // 64-cell segments per row, ceil(nx / 64)
     float4* fields;         // written when write_fields != 0
     float* prev_u;          // outlet plane (x = nx-1) velocities, 3 floats per (y, z)
     TileSlot* slots;
     const float* tau_x;     // tau0 + outlet/inlet sponge per x (wgsl.ts:259-260)
-    uint64_t ns;            // population plane stride (elements)
-    uint32_t plane_bytes;   // ns * element size
-    uint32_t pop_bytes;     // 19 planes (below 4 GiB: buffer addressing)
+    uint64_t ns;            // cells rounded up to 256 (flag array length)
+    uint32_t pop_bytes;     // one population buffer, segment-major (below 4 GiB: buffer addressing)
     uint32_t nx, ny, nz;
     uint32_t tiles_x, tiles_y, ntiles;
     float u_in;
@@ -103,6 +104,8 @@ struct aero_ctx {
     uint32_t* solid;
     float* emit;
     uint8_t* flags;
+    uint32_t* fastmap;      // see StepArgs::fastmap
+    uint32_t nseg, fast_words;
     float4* fields;
     float* prev_u;
     float* temps[2];

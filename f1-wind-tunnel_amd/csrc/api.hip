@@ -73,7 +73,7 @@ int upload_tau(aero_ctx* c) {
 void free_ctx(aero_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    void* ptrs[] = {c->f[0], c->f[1], c->solid, c->emit, c->flags, c->fields, c->prev_u, c->temps[0],
+    void* ptrs[] = {c->f[0], c->f[1], c->solid, c->emit, c->flags, c->fastmap, c->fields, c->prev_u, c->temps[0],
                     c->temps[1], c->tau_x, c->slots, c->red_out, c->car_body, c->car_parts, c->d_rig};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -124,7 +124,8 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) { free_ctx(c); set_global_error(std::string("hipGetDevice: ") + hipGetErrorString(e)); return AERO_E_HIP; }
     c->ns = (n + 255) & ~(uint64_t)255;
-    c->pop_bytes = (size_t)19 * c->ns * store_bytes(storage);
+    // Segment-major populations: 19 x 64 values per 64-cell row segment.
+    c->pop_bytes = (size_t)((grid->nx + 63) / 64) * grid->ny * grid->nz * 19 * 64 * store_bytes(storage);
     if (c->pop_bytes > 0xffffffffull) {
         delete c;
         return fail(nullptr, AERO_E_INVALID, "aero_create: a population array must stay below 4 GiB");
@@ -139,6 +140,8 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
     // (aero_autotune replaces it with a measured choice).
     c->shape = (n > 8000000ull) ? 6 : 0;
     const size_t outlet = (size_t)grid->ny * grid->nz * 3;
+    c->nseg = (grid->nx + 63) / 64;
+    c->fast_words = (uint32_t)(((uint64_t)c->nseg * grid->ny * grid->nz + 31) / 32);
 
     struct Alloc { void** p; size_t bytes; } allocs[] = {
         {&c->f[0], c->pop_bytes},
@@ -146,6 +149,7 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
         {(void**)&c->solid, n * 4},
         {(void**)&c->emit, n * 4},
         {(void**)&c->flags, c->ns},
+        {(void**)&c->fastmap, (size_t)c->fast_words * 4},
         {(void**)&c->fields, n * 16},
         {(void**)&c->prev_u, outlet * 4},
         {(void**)&c->temps[0], n * 4},
@@ -176,13 +180,17 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
             cus = 256;
         c->persist_blocks = (uint32_t)cus * kPersistBlocksPerCU;
     }
-    // Empty tunnel until the first voxelize / set_flags.
-    e = hipMemsetAsync(c->solid, 0, n * 4, c->stream);
+    // Empty tunnel until the first voxelize / set_flags.  The population buffers are
+    // cleared once so that the padding lanes of partial segments hold zeros.
+    e = hipMemsetAsync(c->f[0], 0, c->pop_bytes, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->f[1], 0, c->pop_bytes, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(c->solid, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->emit, 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->flags, 0, c->ns, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->slots, 0, (size_t)c->nslots * sizeof(TileSlot), c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->temps[0], 0, n * 4, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(c->temps[1], 0, n * 4, c->stream);
+    if (e == hipSuccess) e = launch_flags(c);  // flags and fast-segment map of the empty tunnel
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         free_ctx(c);

@@ -188,7 +188,11 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
     constexpr uint32_t ESZ = (uint32_t)sizeof(T);
     const rsrc_t rin = make_rsrc(a_fin, a.pop_bytes);
     const rsrc_t rout = make_rsrc(a_fout, a.pop_bytes);
-    const uint32_t pbytes = a.plane_bytes;  // bytes per population plane
+    // Populations are stored segment-major: the 19 x 64 values of a 64-cell row
+    // segment are contiguous, element (i, x, y, z) at ((seg * 19 + i) * 64 + (x & 63))
+    // with seg = (x >> 6) + nseg * (y + ny * z).  A wavefront owns one segment.
+    constexpr uint32_t SEGB = 19u * 64u * ESZ;  // bytes per segment
+    constexpr uint32_t POPB = 64u * ESZ;        // bytes per population within a segment
 
     const uint32_t txy = a.tiles_x * a.tiles_y;
     const uint32_t tz = l / txy;
@@ -209,10 +213,21 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
     // wavefront holds no solid and no solid-touching cell pull every population with a
     // plain shifted load (inlet/outlet lanes are patched afterwards).  Every other
     // wavefront takes the general path below, which also issues its 19 loads together.
-    const bool yz_inner = (y >= 1) && (y <= NY - 2) && (z >= 1) && (z <= NZ - 2);
+    // The choice is one bit of the fast-segment map (a wave covers one 64-cell row
+    // segment), read with a wave-uniform address; per-cell flags are loaded only by
+    // wavefronts that take the general path.
     const bool first_tile = (tx == 0), last_tile = (tx == a.tiles_x - 1);
-    const uint32_t fl = active ? (uint32_t)*at(a.flags, idx) : 0u;
-    const bool fast = yz_inner && (__builtin_amdgcn_ballot_w64(active && (fl & (FL_GROUP | FL_NEAR)) != 0u) == 0);
+    // y, z and x >> 6 are uniform across the wavefront: its segment index is scalar.
+    const uint32_t seg = (uint32_t)__builtin_amdgcn_readfirstlane(
+        (int)(((uint32_t)x >> 6) + a.nseg * ((uint32_t)y + (uint32_t)NY * (uint32_t)z)));
+    bool fast = false;
+    if (y < NY) {
+        const uint32_t word = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.fastmap[seg >> 5]);
+        fast = ((word >> (seg & 31u)) & 1u) != 0u;
+    }
+    // Byte offset of population 0 of this lane's own cell; population I adds I * POPB.
+    const uint32_t own = seg * SEGB + (uint32_t)lane * ESZ;
+    const uint32_t fl = (!fast && active) ? (uint32_t)*at(a.flags, idx) : 0u;
 
     auto LD = [&](uint32_t voff, uint32_t soff, float w) -> float {
         if constexpr (DEV) return ST::braw(rin, voff, soff);
@@ -234,7 +249,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
             // Solid cell: fout_i = w_i, fields = 0 (wgsl.ts:182-186).
             for19([&](auto ic) {
                 constexpr int I = decltype(ic)::value;
-                SV(idx * ESZ, (uint32_t)I * pbytes, DEV ? 0.0f : wt(I), wt(I));
+                SV(own, (uint32_t)I * POPB, DEV ? 0.0f : wt(I), wt(I));
             });
             if (write_fields) a.fields[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (outlet) { a.prev_u[pidx] = 0.f; a.prev_u[pidx + 1] = 0.f; a.prev_u[pidx + 2] = 0.f; }
@@ -256,10 +271,22 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
 
             float f[19];
             if (fast) {
+                // The source of direction I is the same lane of the segment (cy, cz) rows
+                // back, shifted by one lane for c_x = +-1 (crossing into the neighbouring
+                // segment at lane 0 / lane 63).  Every source lies inside the array:
+                // fast wavefronts are off the floor, roof, side and symmetry rows.
+                const int32_t segb = (int32_t)SEGB, esz = (int32_t)ESZ;
+                const int32_t lp = lane == 0 ? 63 * esz - segb : (lane - 1) * esz;   // c_x = +1
+                const int32_t lm = lane == 63 ? segb : (lane + 1) * esz;              // c_x = -1
+                const int32_t l0 = lane * esz;
+                const int32_t row = (int32_t)a.nseg * segb, plane = (int32_t)(a.nseg * (uint32_t)NY) * segb;
+                const int32_t base = (int32_t)(seg * SEGB);
                 for19([&](auto ic) {
                     constexpr int I = decltype(ic)::value;
-                    const int32_t off = kCX[I] + kCY[I] * NX + kCZ[I] * sxy;
-                    f[I] = LD((uint32_t)((int32_t)idx - off) * ESZ, (uint32_t)I * pbytes, wt(I));
+                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+                    const int32_t lo = cx == 1 ? lp : (cx == -1 ? lm : l0);
+                    const int32_t src = base - cy * row - cz * plane + I * (int32_t)POPB;
+                    f[I] = LD((uint32_t)(src + lo), 0u, wt(I));
                 });
                 if (first_tile) {  // inlet lane x = 0 (wgsl.ts:205)
                     for19([&](auto ic) {
@@ -325,8 +352,12 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                     });
                 }
                 const uint32_t sany = s1 | s2 | s3;
-                const uint32_t own = idx * ESZ;
                 uint32_t voff[19];
+                // byte offset of population p of cell (sx, sy, sz) (only used in range)
+                auto aoff = [&](int sx, int sy, int sz, uint32_t p) -> uint32_t {
+                    const uint32_t sseg = (uint32_t)(sx >> 6) + a.nseg * ((uint32_t)sy + (uint32_t)NY * (uint32_t)sz);
+                    return sseg * SEGB + p * POPB + (uint32_t)(sx & 63) * ESZ;
+                };
                 for19([&](auto ic) {
                     constexpr int I = decltype(ic)::value;
                     constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
@@ -336,9 +367,11 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                         // plane of a regular or mirrored pull
                         uint32_t plane = (uint32_t)I;
                         if constexpr (cz == 1) plane = mir_row ? (uint32_t)kZMIR[I] : (uint32_t)I;
-                        const uint32_t reg = (uint32_t)((int32_t)idx - src_off(ic)) * ESZ + plane * pbytes;
-                        const uint32_t opp = own + (uint32_t)kOPP[I] * pbytes;
-                        const uint32_t dummy = own + (uint32_t)I * pbytes;
+                        int sz = z - cz;
+                        if constexpr (cz == 1) sz = mir_row ? z : z - 1;
+                        const uint32_t reg = aoff(x - cx, y - cy, sz, plane);
+                        const uint32_t opp = own + (uint32_t)kOPP[I] * POPB;
+                        const uint32_t dummy = own + (uint32_t)I * POPB;
                         bool constant = false, bounce = false;
                         if constexpr (cx == 1) constant = constant || (x == 0);
                         if constexpr (cx == -1) constant = constant || outlet;
@@ -469,7 +502,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                     if constexpr (cx * cz != 0) qp = qp + 2.0f * (cx * cz > 0 ? pxz : -pxz);
                     if constexpr (cy * cz != 0) qp = qp + 2.0f * (cy * cz > 0 ? pyz : -pyz);
                     const float v = eq[I] + (om45 * wt(I)) * qp;
-                    ST::bst(rout, idx * ESZ, (uint32_t)I * pbytes, v, wt(I));
+                    ST::bst(rout, own, (uint32_t)I * POPB, v, wt(I));
                 });
             } else {
                 // ---- moment-space regularized collision ----
@@ -562,7 +595,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                 o[17] = kW2 * (ayzm + syzm); o[18] = kW2 * (ayzm - syzm);
                 for19([&](auto ic) {
                     constexpr int I = decltype(ic)::value;
-                    SV(idx * ESZ, (uint32_t)I * pbytes, o[I], wt(I));
+                    SV(own, (uint32_t)I * POPB, o[I], wt(I));
                 });
             }
             if (write_fields) a.fields[idx] = make_float4(ux, uy, uz, rho);
@@ -638,6 +671,13 @@ __global__ void __launch_bounds__(kBlock) k_step_batch(BatchArgs b) {
                             b.write_fields);
 }
 
+// Element of population i of cell idx in the segment-major population layout.
+__host__ __device__ __forceinline__ uint64_t seg_elem(uint64_t idx, uint32_t i, uint32_t nx, uint32_t nseg) {
+    const uint64_t row = idx / nx;  // y + ny * z
+    const uint32_t x = (uint32_t)(idx - row * nx);
+    return (((uint64_t)(x >> 6) + (uint64_t)nseg * row) * 19u + i) * 64u + (x & 63u);
+}
+
 // LBM_INIT (wgsl.ts:147-154) on both buffers: feq(i, 1, (uIn,0,0)) everywhere,
 // fields = (uIn,0,0,1) and the outlet velocity plane = (uIn,0,0).
 template <int S>
@@ -651,8 +691,9 @@ __global__ void __launch_bounds__(256) k_init(StepArgs a, void* fa, void* fb, ui
 #pragma unroll
     for (int i = 0; i < 19; i++) {
         const T v = ST::enc(a.feq_in[i], wt(i));
-        A[(uint64_t)i * a.ns + idx] = v;
-        B[(uint64_t)i * a.ns + idx] = v;
+        const uint64_t e = seg_elem(idx, i, a.nx, a.nseg);
+        A[e] = v;
+        B[e] = v;
     }
     a.fields[idx] = make_float4(a.u_in, 0.f, 0.f, 1.f);
     const uint32_t x = (uint32_t)(idx % a.nx);
@@ -747,20 +788,49 @@ __global__ void __launch_bounds__(256) k_reduce_slots(const TileSlot* __restrict
 }
 
 template <int S>
-__global__ void __launch_bounds__(256) k_pack(const float* __restrict__ in, void* out, uint64_t n, uint64_t ns) {
+__global__ void __launch_bounds__(256) k_pack(const float* __restrict__ in, void* out, uint64_t n, uint32_t nx,
+                                              uint32_t nseg) {
     using ST = Store<S>;
     using T = typename ST::T;
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
-    for (int i = 0; i < 19; i++) ((T*)out)[(uint64_t)i * ns + idx] = ST::enc(in[(uint64_t)i * n + idx], wt(i));
+    for (int i = 0; i < 19; i++) ((T*)out)[seg_elem(idx, i, nx, nseg)] = ST::enc(in[(uint64_t)i * n + idx], wt(i));
 }
 template <int S>
-__global__ void __launch_bounds__(256) k_unpack(const void* in, float* __restrict__ out, uint64_t n, uint64_t ns) {
+__global__ void __launch_bounds__(256) k_unpack(const void* in, float* __restrict__ out, uint64_t n, uint32_t nx,
+                                                uint32_t nseg) {
     using ST = Store<S>;
     using T = typename ST::T;
     const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n) return;
-    for (int i = 0; i < 19; i++) out[(uint64_t)i * n + idx] = ST::ld((const T*)in + (uint64_t)i * ns + idx, wt(i));
+    for (int i = 0; i < 19; i++) out[(uint64_t)i * n + idx] = ST::ld((const T*)in + seg_elem(idx, i, nx, nseg), wt(i));
+}
+
+// Fast-segment map: bit s (s = k + nseg*(y + ny*z)) is set when the 64-cell row
+// segment x in [64k, 64k+63] of row (y, z) lies off the floor, roof, side and symmetry
+// rows and none of its cells is solid, next to a solid, or a duct cell.  A wavefront
+// covers exactly one segment in every launch shape, so it reads one bit instead of
+// 64 flag bytes to choose the plain pull.
+__global__ void __launch_bounds__(256) k_fastmap(const uint8_t* __restrict__ flags, uint32_t* __restrict__ map,
+                                                 uint32_t words, uint32_t nseg, uint32_t nx, uint32_t ny, uint32_t nz) {
+    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    const uint64_t total = (uint64_t)nseg * ny * nz;
+    uint32_t bits = 0u;
+    for (uint32_t b = 0; b < 32u; b++) {
+        const uint64_t s = (uint64_t)w * 32u + b;
+        if (s >= total) break;
+        const uint32_t k = (uint32_t)(s % nseg);
+        const uint32_t y = (uint32_t)((s / nseg) % ny);
+        const uint32_t z = (uint32_t)(s / ((uint64_t)nseg * ny));
+        bool fast = y >= 1u && y + 2u <= ny && z >= 1u && z + 2u <= nz;
+        const uint32_t x1 = (k * 64u + 64u < nx) ? k * 64u + 64u : nx;
+        const uint64_t row = (uint64_t)nx * ((uint64_t)y + (uint64_t)ny * z);
+        for (uint32_t x = k * 64u; fast && x < x1; x++)
+            if (flags[row + x] & (FL_GROUP | FL_NEAR | FL_DUCT)) fast = false;
+        if (fast) bits |= 1u << b;
+    }
+    map[w] = bits;
 }
 
 // ---- host side ----
@@ -774,7 +844,6 @@ static float feq_host(int i, float rho, float ux, float uy, float uz) {
 
 void fill_step_consts(const aero_ctx* c, StepArgs* a) {
     a->ns = c->ns;
-    a->plane_bytes = (uint32_t)(c->ns * (c->storage == AERO_STORE_F32 ? 4u : 2u));
     a->pop_bytes = (uint32_t)c->pop_bytes;
     a->nx = c->grid.nx; a->ny = c->grid.ny; a->nz = c->grid.nz;
     a->u_in = c->sim.u_in;
@@ -785,6 +854,8 @@ void fill_step_consts(const aero_ctx* c, StepArgs* a) {
         a->floor_term[i] = 6.0f * wt(i) * (float)kCX[i] * c->sim.u_in;
     }
     a->flags = c->flags;
+    a->fastmap = c->fastmap;
+    a->nseg = c->nseg;
     a->fields = c->fields;
     a->prev_u = c->prev_u;
     a->slots = c->slots;
@@ -808,6 +879,8 @@ hipError_t launch_flags(aero_ctx* c) {
     const uint64_t n = c->grid.n;
     hipLaunchKernelGGL(k_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->solid, c->emit,
                        c->flags, c->grid.nx, c->grid.ny, c->grid.nz);
+    hipLaunchKernelGGL(k_fastmap, dim3((c->fast_words + 255) / 256), dim3(256), 0, c->stream, c->flags, c->fastmap,
+                       c->fast_words, c->nseg, c->grid.nx, c->grid.ny, c->grid.nz);
     return hipGetLastError();
 }
 
@@ -917,9 +990,9 @@ hipError_t launch_pack_pops(aero_ctx* c, const float* d_in) {
     const dim3 grid((unsigned)((n + 255) / 256));
     void* out = c->f[c->cur];
     switch (c->storage) {
-    case AERO_STORE_F32: hipLaunchKernelGGL(k_pack<AERO_STORE_F32>, grid, dim3(256), 0, c->stream, d_in, out, n, c->ns); break;
-    case AERO_STORE_F16S: hipLaunchKernelGGL(k_pack<AERO_STORE_F16S>, grid, dim3(256), 0, c->stream, d_in, out, n, c->ns); break;
-    default: hipLaunchKernelGGL(k_pack<AERO_STORE_BF16S>, grid, dim3(256), 0, c->stream, d_in, out, n, c->ns); break;
+    case AERO_STORE_F32: hipLaunchKernelGGL(k_pack<AERO_STORE_F32>, grid, dim3(256), 0, c->stream, d_in, out, n, c->grid.nx, c->nseg); break;
+    case AERO_STORE_F16S: hipLaunchKernelGGL(k_pack<AERO_STORE_F16S>, grid, dim3(256), 0, c->stream, d_in, out, n, c->grid.nx, c->nseg); break;
+    default: hipLaunchKernelGGL(k_pack<AERO_STORE_BF16S>, grid, dim3(256), 0, c->stream, d_in, out, n, c->grid.nx, c->nseg); break;
     }
     return hipGetLastError();
 }
@@ -929,9 +1002,9 @@ hipError_t launch_unpack_pops(aero_ctx* c, float* d_out) {
     const dim3 grid((unsigned)((n + 255) / 256));
     const void* in = c->f[c->cur];
     switch (c->storage) {
-    case AERO_STORE_F32: hipLaunchKernelGGL(k_unpack<AERO_STORE_F32>, grid, dim3(256), 0, c->stream, in, d_out, n, c->ns); break;
-    case AERO_STORE_F16S: hipLaunchKernelGGL(k_unpack<AERO_STORE_F16S>, grid, dim3(256), 0, c->stream, in, d_out, n, c->ns); break;
-    default: hipLaunchKernelGGL(k_unpack<AERO_STORE_BF16S>, grid, dim3(256), 0, c->stream, in, d_out, n, c->ns); break;
+    case AERO_STORE_F32: hipLaunchKernelGGL(k_unpack<AERO_STORE_F32>, grid, dim3(256), 0, c->stream, in, d_out, n, c->grid.nx, c->nseg); break;
+    case AERO_STORE_F16S: hipLaunchKernelGGL(k_unpack<AERO_STORE_F16S>, grid, dim3(256), 0, c->stream, in, d_out, n, c->grid.nx, c->nseg); break;
+    default: hipLaunchKernelGGL(k_unpack<AERO_STORE_BF16S>, grid, dim3(256), 0, c->stream, in, d_out, n, c->grid.nx, c->nseg); break;
     }
     return hipGetLastError();
 }
