@@ -18,16 +18,18 @@ struct HeatArgs {
     float dt, cool, diffuse;
 };
 
-__device__ __forceinline__ uint64_t heat_idx(const HeatArgs& h, int x, int y, int z) {
+// Cell index with the reference's addressing rules: z < 0 mirrors to -z-1, then every
+// coordinate is clamped into the grid (wgsl.ts:486-492).  32-bit: grids stay below 2^31
+// cells.
+__device__ __forceinline__ uint32_t heat_idx(const HeatArgs& h, int x, int y, int z) {
     if (z < 0) z = -z - 1;
     x = x < 0 ? 0 : (x > (int)h.nx - 1 ? (int)h.nx - 1 : x);
     y = y < 0 ? 0 : (y > (int)h.ny - 1 ? (int)h.ny - 1 : y);
-    z = z < 0 ? 0 : (z > (int)h.nz - 1 ? (int)h.nz - 1 : z);
-    return (uint64_t)x + (uint64_t)h.nx * ((uint64_t)y + (uint64_t)h.ny * (uint64_t)z);
+    z = z > (int)h.nz - 1 ? (int)h.nz - 1 : z;
+    return (uint32_t)x + h.nx * ((uint32_t)y + h.ny * (uint32_t)z);
 }
 __device__ __forceinline__ float heat_at(const HeatArgs& h, int x, int y, int z) {
-    if (x < 0) return 0.0f;
-    return h.tin[heat_idx(h, x, y, z)];
+    return x < 0 ? 0.0f : h.tin[heat_idx(h, x, y, z)];
 }
 
 __global__ void __launch_bounds__(256) k_heat(HeatArgs h) {
@@ -35,26 +37,35 @@ __global__ void __launch_bounds__(256) k_heat(HeatArgs h) {
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     const int z = blockIdx.z;
     if (x >= (int)h.nx || y >= (int)h.ny) return;
-    const uint64_t idx = (uint64_t)x + (uint64_t)h.nx * ((uint64_t)y + (uint64_t)h.ny * (uint64_t)z);
+    const uint32_t idx = (uint32_t)x + h.nx * ((uint32_t)y + h.ny * (uint32_t)z);
+    // Loads that do not depend on the velocity go out first, together with it: the
+    // six neighbour temperatures and the flag byte.
+    const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    float nbv[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) nbv[k] = heat_at(h, x + off[k][0], y + off[k][1], z + off[k][2]);
+    const uint32_t fl = h.flags[idx];
     const float4 fv = h.fields[idx];
-    if (fv.w <= 0.0f) { h.tout[idx] = h.emit[idx]; return; }
+    if (fv.w <= 0.0f) { __builtin_nontemporal_store(h.emit[idx], h.tout + idx); return; }
     const float gx = (float)x - fv.x * h.dt, gy = (float)y - fv.y * h.dt, gz = (float)z - fv.z * h.dt;
     const float flx = __builtin_floorf(gx), fly = __builtin_floorf(gy), flz = __builtin_floorf(gz);
     const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
     const float fx = gx - (float)ix, fy = gy - (float)iy, fz = gz - (float)iz;
+    float tv[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) tv[k] = heat_at(h, ix + (k & 1), iy + ((k >> 1) & 1), iz + ((k >> 2) & 1));
     float t = 0.0f;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
         const float w = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy) * (oz ? fz : 1.0f - fz);
-        t = t + heat_at(h, ix + ox, iy + oy, iz + oz) * w;
+        t = t + tv[k] * w;
     }
-    const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
     float nb = 0.0f;
 #pragma unroll
-    for (int k = 0; k < 6; k++) nb = nb + heat_at(h, x + off[k][0], y + off[k][1], z + off[k][2]);
+    for (int k = 0; k < 6; k++) nb = nb + nbv[k];
     float src = 0.0f;
-    if (h.flags[idx] & FL_HOTNB) {
+    if (fl & FL_HOTNB) {
 #pragma unroll
         for (int k = 0; k < 6; k++) {
             const int cx = x + off[k][0], cy = y + off[k][1], cz = z + off[k][2];
@@ -67,7 +78,7 @@ __global__ void __launch_bounds__(256) k_heat(HeatArgs h) {
     t = t * (1.0f - h.diffuse) + (nb / 6.0f) * h.diffuse;
     if (src > 0.0f) t = t * (1.0f - 0.35f) + src * 0.35f;
     const float o = t * (1.0f - h.cool);
-    h.tout[idx] = o > 0.0f ? o : 0.0f;
+    __builtin_nontemporal_store(o > 0.0f ? o : 0.0f, h.tout + idx);
 }
 
 hipError_t launch_heat(aero_ctx* c, const aero_heat* hp) {

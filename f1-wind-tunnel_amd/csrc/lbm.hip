@@ -813,24 +813,26 @@ __global__ void __launch_bounds__(256) k_unpack(const void* in, float* __restric
 // 64 flag bytes to choose the plain pull.
 __global__ void __launch_bounds__(256) k_fastmap(const uint8_t* __restrict__ flags, uint32_t* __restrict__ map,
                                                  uint32_t words, uint32_t nseg, uint32_t nx, uint32_t ny, uint32_t nz) {
-    const uint32_t w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= words) return;
+    // One thread per segment; a wavefront's 64 segment bits are words 2w and 2w + 1.
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t total = (uint64_t)nseg * ny * nz;
-    uint32_t bits = 0u;
-    for (uint32_t b = 0; b < 32u; b++) {
-        const uint64_t s = (uint64_t)w * 32u + b;
-        if (s >= total) break;
+    bool fast = false;
+    if (s < total) {
         const uint32_t k = (uint32_t)(s % nseg);
         const uint32_t y = (uint32_t)((s / nseg) % ny);
         const uint32_t z = (uint32_t)(s / ((uint64_t)nseg * ny));
-        bool fast = y >= 1u && y + 2u <= ny && z >= 1u && z + 2u <= nz;
+        fast = y >= 1u && y + 2u <= ny && z >= 1u && z + 2u <= nz;
         const uint32_t x1 = (k * 64u + 64u < nx) ? k * 64u + 64u : nx;
         const uint64_t row = (uint64_t)nx * ((uint64_t)y + (uint64_t)ny * z);
         for (uint32_t x = k * 64u; fast && x < x1; x++)
             if (flags[row + x] & (FL_GROUP | FL_NEAR | FL_DUCT)) fast = false;
-        if (fast) bits |= 1u << b;
     }
-    map[w] = bits;
+    const uint64_t bits = __builtin_amdgcn_ballot_w64(fast);
+    const uint64_t w0 = s >> 5;  // s is a multiple of 64 in lane 0
+    if ((threadIdx.x & 63) == 0) {
+        if (w0 < words) map[w0] = (uint32_t)bits;
+        if (w0 + 1 < words) map[w0 + 1] = (uint32_t)(bits >> 32);
+    }
 }
 
 // ---- host side ----
@@ -879,7 +881,8 @@ hipError_t launch_flags(aero_ctx* c) {
     const uint64_t n = c->grid.n;
     hipLaunchKernelGGL(k_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->solid, c->emit,
                        c->flags, c->grid.nx, c->grid.ny, c->grid.nz);
-    hipLaunchKernelGGL(k_fastmap, dim3((c->fast_words + 255) / 256), dim3(256), 0, c->stream, c->flags, c->fastmap,
+    const uint64_t segs = (uint64_t)c->nseg * c->grid.ny * c->grid.nz;
+    hipLaunchKernelGGL(k_fastmap, dim3((unsigned)((segs + 255) / 256)), dim3(256), 0, c->stream, c->flags, c->fastmap,
                        c->fast_words, c->nseg, c->grid.nx, c->grid.ny, c->grid.nz);
     return hipGetLastError();
 }
