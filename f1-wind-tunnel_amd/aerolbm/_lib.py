@@ -111,6 +111,7 @@ _PROTOS = {
     "aero_batch_autotune": (C.c_int, [_P, _U32, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "aero_batch_set_launch_shape": (C.c_int, [_P, C.c_int]),
     "aero_batch_get_launch_shape": (C.c_int, [_P, C.POINTER(C.c_int)]),
+    "aero_batch_set_group_bytes": (C.c_int, [_P, C.c_uint64]),
     "aero_mlp_create": (C.c_int, [_U32, _U32, _P, _U32, C.POINTER(_P)]),
     "aero_mlp_destroy": (C.c_int, [_P]),
     "aero_mlp_last_error": (C.c_char_p, [_P]),

@@ -94,12 +94,21 @@ def sequential(points: Sequence[Workload], settle: int, measure: int, storage: s
     return MapRun(out, wall, None, storage, cells, settle + measure)
 
 
+# Population bytes per instance group (0: every instance in each launch).  Groups sized
+# to the Infinity Cache were measured slower on this map (192 MB: 465 ms against 400 ms
+# of device time for 25 tunnels x 2000 steps): the smaller launches lose more to fill
+# and drain than the cache residency gains.
+GROUP_BYTES = 0
+
+
 def batched(points: Sequence[Workload], settle: int, measure: int, storage: str = "f16s", math: str = "fast",
-            steps_per_graph: int = 100, autotune: bool = True, launch_shape: Optional[int] = None) -> MapRun:
+            steps_per_graph: int = 100, autotune: bool = True, launch_shape: Optional[int] = None,
+            group_bytes: int = GROUP_BYTES) -> MapRun:
     """All points in one AeroBatch; settle and measure phases replayed from graphs."""
     t0 = time.perf_counter()
     b = AeroBatch([w.engine_kwargs() for w in points], storage=storage, math=math)
     try:
+        b.set_group_bytes(group_bytes)
         for e, w in zip(b.engines, points):
             e.set_shape(w.shape, prims=w.prims)
             e.reset()

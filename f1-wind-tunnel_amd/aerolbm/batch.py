@@ -100,6 +100,11 @@ class AeroBatch:
     def set_launch_shape(self, shape: int) -> None:
         self._c(self._lib.aero_batch_set_launch_shape(self.b, int(shape)), "aero_batch_set_launch_shape")
 
+    def set_group_bytes(self, nbytes: int) -> None:
+        """Instance groups sized to nbytes of populations (0: one group); see
+        aero_batch_set_group_bytes."""
+        self._c(self._lib.aero_batch_set_group_bytes(self.b, int(nbytes)), "aero_batch_set_group_bytes")
+
     def launch_shape(self) -> int:
         sh = C.c_int()
         self._c(self._lib.aero_batch_get_launch_shape(self.b, C.byref(sh)), "aero_batch_get_launch_shape")

@@ -1,6 +1,11 @@
 // Batched tunnels (aero_batch_*): many independent tunnels of any sizes stepped by one
 // fused kernel launch per step, replayed from a captured HIP graph.
 //
+// Instance groups: with a byte budget set (aero_batch_set_group_bytes), consecutive
+// instances are grouped so that each group's two population buffers fit the budget,
+// and a run advances one group by all its steps before the next.  Sized to the
+// 256 MB Infinity Cache, a group's working set then stays in it from step to step.
+//
 // The reference runs a ride-height sweep as 25 sequential settle/measure periods on
 // one tunnel (src/ui/aeromap.ts:73-114).  Here each sweep point is its own instance
 // context (created, voxelized, initialised and read through the ordinary aero_* calls
@@ -21,8 +26,10 @@ struct aero_batch {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     StepArgs* d_tab = nullptr;
-    uint32_t* d_begin = nullptr;
-    uint32_t total_tiles = 0;
+    uint32_t* d_begin = nullptr;     // per group: count + 1 tile offsets
+    struct Group { uint32_t first, count, begin_off, tiles; };
+    std::vector<Group> groups;
+    uint64_t group_bytes = 0;        // 0: one group holding every instance
     // cached graph of `graph_steps` fused steps (even, so parity is restored)
     hipGraphExec_t gexec = nullptr;
     uint32_t graph_steps = 0;
@@ -53,14 +60,16 @@ void drop_graph(aero_batch* b) {
     b->graph_shape = -1;
 }
 
-// Uploads the instance table and tile offsets for the current shape.  Every
-// instance must be at the same ping-pong parity; returns that parity in *parity.
+// Uploads the instance table and the per-group tile offsets for the current shape.
+// Every instance must be at the same ping-pong parity; returns that parity in *parity.
 int prepare(aero_batch* b, int* parity) {
     const int p0 = b->inst[0]->cur;
-    std::vector<StepArgs> tab(b->inst.size());
-    std::vector<uint32_t> begin(b->inst.size() + 1);
-    uint32_t acc = 0;
-    for (size_t i = 0; i < b->inst.size(); i++) {
+    const size_t n = b->inst.size();
+    std::vector<StepArgs> tab(n);
+    std::vector<uint32_t> begin;
+    b->groups.clear();
+    uint64_t bytes = 0;
+    for (size_t i = 0; i < n; i++) {
         aero_ctx* c = b->inst[i];
         if (c->cur != p0) return bfail(b, AERO_E_STATE, "aero_batch: instances are at different step parities");
         BCHK(b, hipStreamSynchronize(c->stream));
@@ -70,11 +79,19 @@ int prepare(aero_batch* b, int* parity) {
         a.fout = c->f[1];
         tile_geometry(c, b->shape, &a.tiles_x, &a.tiles_y, &a.ntiles);
         tab[i] = a;
-        begin[i] = acc;
-        acc += a.ntiles;
+        const uint64_t ib = 2ull * c->pop_bytes;
+        if (b->groups.empty() || (b->group_bytes && bytes + ib > b->group_bytes)) {
+            if (!b->groups.empty()) begin.push_back(b->groups.back().tiles);
+            b->groups.push_back({(uint32_t)i, 0u, (uint32_t)begin.size(), 0u});
+            bytes = 0;
+        }
+        aero_batch::Group& g = b->groups.back();
+        begin.push_back(g.tiles);
+        g.tiles += a.ntiles;
+        g.count++;
+        bytes += ib;
     }
-    begin[b->inst.size()] = acc;
-    b->total_tiles = acc;
+    begin.push_back(b->groups.back().tiles);
     BCHK(b, hipMemcpyAsync(b->d_tab, tab.data(), tab.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
     BCHK(b, hipMemcpyAsync(b->d_begin, begin.data(), begin.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
     BCHK(b, hipStreamSynchronize(b->stream));  // the host tables go out of scope here
@@ -82,11 +99,11 @@ int prepare(aero_batch* b, int* parity) {
     return AERO_OK;
 }
 
-BatchArgs args(aero_batch* b, int parity, int write_fields) {
+BatchArgs args(aero_batch* b, const aero_batch::Group& g, int parity, int write_fields) {
     BatchArgs a{};
-    a.tab = b->d_tab;
-    a.begin = b->d_begin;
-    a.ninst = (uint32_t)b->inst.size();
+    a.tab = b->d_tab + g.first;
+    a.begin = b->d_begin + g.begin_off;
+    a.ninst = g.count;
     a.parity = parity;
     a.write_fields = write_fields;
     return a;
@@ -99,12 +116,14 @@ void advance(aero_batch* b, uint32_t nsteps) {
     }
 }
 
+// nsteps fused steps of every group, group after group.
 int launch_eager(aero_batch* b, int parity, uint32_t nsteps, bool fields_last) {
-    for (uint32_t k = 0; k < nsteps; k++) {
-        const int wf = (fields_last && k + 1 == nsteps) ? 1 : 0;
-        BCHK(b, launch_batch(args(b, parity ^ (int)(k & 1u), wf), b->storage, b->exact, b->shape, b->total_tiles,
-                             b->stream));
-    }
+    for (const aero_batch::Group& g : b->groups)
+        for (uint32_t k = 0; k < nsteps; k++) {
+            const int wf = (fields_last && k + 1 == nsteps) ? 1 : 0;
+            BCHK(b, launch_batch(args(b, g, parity ^ (int)(k & 1u), wf), b->storage, b->exact, b->shape, g.tiles,
+                                 b->stream));
+        }
     return AERO_OK;
 }
 
@@ -137,7 +156,7 @@ int aero_batch_create(uint32_t n, const aero_grid* grids, const aero_sim* sims, 
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e == hipSuccess) e = hipMalloc(&b->d_tab, n * sizeof(StepArgs));
-    if (e == hipSuccess) e = hipMalloc(&b->d_begin, (n + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->d_begin, (2 * (size_t)n + 1) * sizeof(uint32_t));
     if (e != hipSuccess) {
         for (aero_ctx* x : b->inst) aero_destroy(x);
         if (b->d_tab) (void)hipFree(b->d_tab);
@@ -180,6 +199,13 @@ int aero_batch_set_launch_shape(aero_batch* b, int shape) {
         return bfail(b, AERO_E_INVALID, "aero_batch_set_launch_shape: shape must be one of 0,1,4,5,6,7");
     if (shape != b->shape) drop_graph(b);
     b->shape = shape;
+    return AERO_OK;
+}
+
+int aero_batch_set_group_bytes(aero_batch* b, uint64_t bytes) {
+    if (!b) return AERO_E_INVALID;
+    if (bytes != b->group_bytes) drop_graph(b);
+    b->group_bytes = bytes;
     return AERO_OK;
 }
 

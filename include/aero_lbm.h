@@ -215,6 +215,12 @@ int aero_batch_run_graph(aero_batch* batch, uint32_t nsteps, uint32_t steps_per_
 int aero_batch_autotune(aero_batch* batch, uint32_t steps_per_shape, int* shape, float* us_per_step);
 int aero_batch_set_launch_shape(aero_batch* batch, int shape);
 int aero_batch_get_launch_shape(aero_batch* batch, int* shape);
+/* Instance groups: consecutive instances whose two population buffers together fit
+ * in `bytes` form a group, and every run (eager or graph) advances one group by all
+ * its steps before the next, so a group sized to the 256 MB Infinity Cache keeps its
+ * working set there across steps.  0 (the default) keeps all instances in one launch
+ * per step.  Results are identical either way; only the order of work changes. */
+int aero_batch_set_group_bytes(aero_batch* batch, uint64_t bytes);
 
 /* ---- Surrogate MLP ensemble ----
  * No counterpart in the reference: there every setup-to-force answer is an LBM run

@@ -249,3 +249,21 @@ def test_aeromap_window_forces_are_rounding_limited_against_fp64():
     print("  window error vs FP64:", {k: f"{v:.2e}" for k, v in res.items()}, flush=True)
     assert max(res.values()) <= 0.10
     assert res["f16s"] <= 2.0 * max(res["f32 exact"], res["f32 exact + 1e-7"]) + 0.005
+
+
+def test_instance_groups_give_identical_results():
+    """Group budgets that split the three tunnels one per group, two + one, or keep
+    them together all give bit-identical populations and forces, eager and graph."""
+    runs = []
+    for gb in (0, 1, 2 * 2 * 64 * 24 * 16 * 19 * 2 + 1):
+        b = AeroBatch(SPECS, storage="f16s")
+        setup(b.engines)
+        b.set_launch_shape(0)
+        b.set_group_bytes(gb)
+        b.run_graph(10, 4)
+        b.step(3)
+        runs.append([(e.populations(), e.read_forces_raw()) for e in b.engines])
+        b.close()
+    for other in runs[1:]:
+        for (p0, (d0, q0, n0)), (p1, (d1, q1, n1)) in zip(runs[0], other):
+            assert np.array_equal(p0, p1) and np.array_equal(d0, d1) and np.array_equal(q0, q1) and n0 == n1
