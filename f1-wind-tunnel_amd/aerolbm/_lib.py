@@ -66,8 +66,30 @@ class Rig(C.Structure):
                 ("extra", C.c_float * 4), ("fly", C.c_float * 4), ("grids", (C.c_float * 4) * 12)]
 
 
+class View(C.Structure):
+    """aero_view == WGSL struct View (src/engine/wgsl.ts:431-434)."""
+    _fields_ = [("u_in", C.c_float), ("v_real", C.c_float), ("cp_min", C.c_float), ("cp_max", C.c_float),
+                ("t_min", C.c_float), ("t_max", C.c_float), ("speed_max", C.c_float), ("vort_max", C.c_float)]
+
+
+class PSim(C.Structure):
+    """aero_psim == WGSL struct PSim (src/engine/wgsl.ts:518-523)."""
+    _fields_ = [("count", C.c_uint32), ("spawn_mode", C.c_uint32), ("color_mode", C.c_uint32),
+                ("frame", C.c_uint32), ("dt", C.c_float), ("life", C.c_float), ("scatter", C.c_uint32),
+                ("rake_rows", C.c_uint32), ("spawn_min", C.c_float * 3), ("rake_cols", C.c_uint32),
+                ("spawn_max", C.c_float * 3), ("pad", C.c_float)]
+
+
+class Stream(C.Structure):
+    """aero_stream == WGSL struct Stream (src/engine/render-wgsl.ts:521-526)."""
+    _fields_ = [("lines", C.c_uint32), ("steps", C.c_uint32), ("seed_mode", C.c_uint32), ("color_mode", C.c_uint32),
+                ("seed_min", C.c_float * 3), ("rows", C.c_uint32), ("seed_max", C.c_float * 3), ("cols", C.c_uint32),
+                ("step_len", C.c_float), ("time", C.c_float), ("per_emitter", C.c_uint32), ("emitters", C.c_uint32)]
+
+
 assert C.sizeof(Grid) == 32 and C.sizeof(Sim) == 16 and C.sizeof(Heat) == 16
 assert C.sizeof(Shape) == 96 and C.sizeof(Prim) == 64
+assert C.sizeof(View) == 32 and C.sizeof(PSim) == 64 and C.sizeof(Stream) == 64
 
 _P = C.c_void_p
 _U32 = C.c_uint32
@@ -95,6 +117,13 @@ _PROTOS = {
     "aero_write_populations": (C.c_int, [_P, _P]),
     "aero_guard_count": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "aero_sync": (C.c_int, [_P]),
+    "aero_write_fields": (C.c_int, [_P, _P]),
+    "aero_sample_flow": (C.c_int, [_P, _P, _U32, C.POINTER(View), _U32, _P]),
+    "aero_particles_alloc": (C.c_int, [_P, _U32]),
+    "aero_particles_step": (C.c_int, [_P, C.POINTER(PSim), C.POINTER(View)]),
+    "aero_particles_read": (C.c_int, [_P, _P]),
+    "aero_particles_write": (C.c_int, [_P, _P]),
+    "aero_streamlines": (C.c_int, [_P, C.POINTER(Stream), C.POINTER(View), _P, _P]),
     "aero_profile_begin": (C.c_int, [_P, _U32]),
     "aero_autotune": (C.c_int, [_P, _U32, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     "aero_set_launch_shape": (C.c_int, [_P, C.c_int]),

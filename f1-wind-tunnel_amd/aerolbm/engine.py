@@ -385,3 +385,51 @@ class HipAeroEngine:
 
     def sync(self) -> None:
         self._c(self._lib.aero_sync(self.ctx), "aero_sync")
+
+    def write_fields(self, f: np.ndarray) -> None:
+        """Replaces the (ux, uy, uz, rho) field; rho <= 0 marks solid cells for the
+        field consumers."""
+        a = np.ascontiguousarray(f, dtype=np.float32).reshape(self.n, 4)
+        self._c(self._lib.aero_write_fields(self.ctx, ptr(a)), "aero_write_fields")
+
+    # ---- field consumers (src/engine/wgsl.ts:324-457,517-595; render-wgsl.ts:520-576) ----
+
+    def sample_flow(self, points: np.ndarray, view: "_lib.View", color_mode: int = 0) -> np.ndarray:
+        """(n, 8): ux, uy, uz, rho, temp, fluid weight, vorticityAt, fieldValue."""
+        pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+        out = np.empty((pts.shape[0], 8), np.float32)
+        self._c(self._lib.aero_sample_flow(self.ctx, ptr(pts), pts.shape[0], C.byref(view), int(color_mode), ptr(out)),
+                "aero_sample_flow")
+        return out
+
+    def particles_alloc(self, count: int) -> None:
+        """The zero-filled particle buffer (engine.ts:243)."""
+        self._c(self._lib.aero_particles_alloc(self.ctx, int(count)), "aero_particles_alloc")
+        self.particle_count = int(count)
+
+    def particles_step(self, psim: "_lib.PSim", view: "_lib.View") -> None:
+        """One PARTICLES_SIM dispatch (wgsl.ts:569-594)."""
+        self._c(self._lib.aero_particles_step(self.ctx, C.byref(psim), C.byref(view)), "aero_particles_step")
+
+    def particles(self) -> np.ndarray:
+        """(count, 8): pos x, y, z, age, step dx, dy, dz, colour value."""
+        out = np.empty((getattr(self, "particle_count", 0), 8), np.float32)
+        if out.size:
+            self._c(self._lib.aero_particles_read(self.ctx, ptr(out)), "aero_particles_read")
+        return out
+
+    def write_particles(self, p: np.ndarray) -> None:
+        a = np.ascontiguousarray(p, dtype=np.float32).reshape(getattr(self, "particle_count", 0), 8)
+        self._c(self._lib.aero_particles_write(self.ctx, ptr(a)), "aero_particles_write")
+
+    def streamlines(self, stream: "_lib.Stream", view: "_lib.View", emitters=None) -> np.ndarray:
+        """One STREAM_SIM dispatch (render-wgsl.ts:551-575): (lines, steps, 4) points,
+        the fourth component the colour value or -1 after the line died."""
+        em = np.zeros((8, 4), np.float32)
+        if emitters is not None:
+            e = np.asarray(emitters, np.float32).reshape(-1, 4)
+            em[:e.shape[0]] = e
+        out = np.empty((stream.lines, stream.steps, 4), np.float32)
+        self._c(self._lib.aero_streamlines(self.ctx, C.byref(stream), C.byref(view), ptr(em), ptr(out)),
+                "aero_streamlines")
+        return out

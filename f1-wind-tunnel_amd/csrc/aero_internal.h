@@ -124,6 +124,13 @@ struct aero_ctx {
     aero_rig rig;
     aero_rig* d_rig;  // device copy read by the voxelizer
     int has_car;
+    // field consumers: particle buffer (2 float4 per particle), streamline points,
+    // the 8 emitters of the vortex-ring seeds
+    float* parts;
+    uint32_t parts_cap;
+    float* stream_pts;
+    size_t stream_cap;      // float4 points
+    float* d_emitters;
     // Live timing of stream-collide bursts (aero_profile_begin/end): one event pair
     // per aero_step call, recorded on the context stream.
     std::vector<hipEvent_t> prof_ev;
@@ -145,6 +152,9 @@ bool batch_shape_ok(int shape);
 hipError_t launch_batch(const BatchArgs& b, int storage, int exact, int shape, uint32_t total_tiles, hipStream_t st);
 hipError_t launch_reduce_slots(aero_ctx* c);
 hipError_t launch_heat(aero_ctx* c, const aero_heat* h);
+hipError_t launch_probe(aero_ctx* c, const aero_view* v, uint32_t mode, const float* d_pts, uint32_t n, float* d_out);
+hipError_t launch_particles(aero_ctx* c, const aero_psim* ps, const aero_view* v);
+hipError_t launch_streams(aero_ctx* c, const aero_stream* st, const aero_view* v, const float* d_em);
 hipError_t launch_voxelize(aero_ctx* c, const aero_shape* s, const float* d_profile,
                            const aero_prim* d_prims, uint32_t nprims, const float* host_extra);
 hipError_t launch_pack_pops(aero_ctx* c, const float* d_in);    // f32 SoA -> storage

@@ -74,7 +74,8 @@ void free_ctx(aero_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     void* ptrs[] = {c->f[0], c->f[1], c->solid, c->emit, c->flags, c->fastmap, c->fields, c->prev_u, c->temps[0],
-                    c->temps[1], c->tau_x, c->slots, c->red_out, c->car_body, c->car_parts, c->d_rig};
+                    c->temps[1], c->tau_x, c->slots, c->red_out, c->car_body, c->car_parts, c->d_rig,
+                    c->parts, c->stream_pts, c->d_emitters};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t ev : c->prof_ev) (void)hipEventDestroy(ev);
@@ -606,6 +607,108 @@ int aero_write_populations(aero_ctx* c, const float* in) {
 
 int aero_sync(aero_ctx* c) {
     CTXCHK(c);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AERO_OK;
+}
+
+int aero_write_fields(aero_ctx* c, const float* in) {
+    CTXCHK(c);
+    if (!in) return fail(c, AERO_E_INVALID, "aero_write_fields: null input");
+    HIPCHK(c, hipMemcpyAsync(c->fields, in, c->grid.n * 16, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AERO_OK;
+}
+
+// ---- field consumers ----
+
+int aero_sample_flow(aero_ctx* c, const float* pts, uint32_t n, const aero_view* view, uint32_t mode, float* out) {
+    CTXCHK(c);
+    if (!pts || !view || !out) return fail(c, AERO_E_INVALID, "aero_sample_flow: null argument");
+    if (n > (1u << 26)) return fail(c, AERO_E_INVALID, "aero_sample_flow: more than 2^26 points");
+    if (n == 0) return AERO_OK;
+    float *d_p = nullptr, *d_o = nullptr;
+    hipError_t e = hipMalloc(&d_p, (size_t)n * 12);
+    if (e == hipSuccess) e = hipMalloc(&d_o, (size_t)n * 32);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_p, pts, (size_t)n * 12, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = launch_probe(c, view, mode, d_p, n, d_o);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, d_o, (size_t)n * 32, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (d_p) (void)hipFree(d_p);
+    if (d_o) (void)hipFree(d_o);
+    if (e != hipSuccess) return hip_fail(c, e, "aero_sample_flow");
+    return AERO_OK;
+}
+
+int aero_particles_alloc(aero_ctx* c, uint32_t count) {
+    CTXCHK(c);
+    if (count > (1u << 26)) return fail(c, AERO_E_INVALID, "aero_particles_alloc: more than 2^26 particles");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->parts) (void)hipFree(c->parts);
+    c->parts = nullptr;
+    c->parts_cap = 0;
+    if (count == 0) return AERO_OK;
+    HIPCHK(c, hipMalloc(&c->parts, (size_t)count * 32));
+    HIPCHK(c, hipMemsetAsync(c->parts, 0, (size_t)count * 32, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->parts_cap = count;
+    return AERO_OK;
+}
+
+int aero_particles_step(aero_ctx* c, const aero_psim* ps, const aero_view* view) {
+    CTXCHK(c);
+    if (!ps || !view) return fail(c, AERO_E_INVALID, "aero_particles_step: null argument");
+    if (ps->count > c->parts_cap) return fail(c, AERO_E_INVALID, "aero_particles_step: count exceeds the allocation");
+    if (ps->spawn_mode == 1u && (ps->rake_rows == 0 || ps->rake_cols == 0 ||
+                                 (uint64_t)ps->rake_rows * ps->rake_cols > 0xffffffffull))
+        return fail(c, AERO_E_INVALID, "aero_particles_step: rake spawning needs 1 <= rake_rows * rake_cols < 2^32");
+    HIPCHK(c, launch_particles(c, ps, view));
+    return AERO_OK;
+}
+
+int aero_particles_read(aero_ctx* c, float* out) {
+    CTXCHK(c);
+    if (!out) return fail(c, AERO_E_INVALID, "aero_particles_read: null output");
+    if (!c->parts_cap) return AERO_OK;
+    HIPCHK(c, hipMemcpyAsync(out, c->parts, (size_t)c->parts_cap * 32, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AERO_OK;
+}
+
+int aero_particles_write(aero_ctx* c, const float* in) {
+    CTXCHK(c);
+    if (!in) return fail(c, AERO_E_INVALID, "aero_particles_write: null input");
+    if (!c->parts_cap) return AERO_OK;
+    HIPCHK(c, hipMemcpyAsync(c->parts, in, (size_t)c->parts_cap * 32, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return AERO_OK;
+}
+
+int aero_streamlines(aero_ctx* c, const aero_stream* st, const aero_view* view, const float* emitters, float* out) {
+    CTXCHK(c);
+    if (!st || !view) return fail(c, AERO_E_INVALID, "aero_streamlines: null argument");
+    const uint64_t npts = (uint64_t)st->lines * st->steps;
+    if (npts > (1ull << 26)) return fail(c, AERO_E_INVALID, "aero_streamlines: more than 2^26 points");
+    if (st->seed_mode == 1u) {
+        if (!emitters) return fail(c, AERO_E_INVALID, "aero_streamlines: emitter seeding needs the emitters");
+        if (st->emitters < 1 || st->emitters > 8 || st->per_emitter < 1)
+            return fail(c, AERO_E_INVALID, "aero_streamlines: emitter seeding needs 1..8 emitters, per_emitter >= 1");
+    } else if (st->rows < 1 || st->cols < 1) {
+        return fail(c, AERO_E_INVALID, "aero_streamlines: rake seeding needs rows, cols >= 1");
+    }
+    if (npts == 0) return AERO_OK;
+    if (npts > c->stream_cap) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (c->stream_pts) (void)hipFree(c->stream_pts);
+        c->stream_pts = nullptr;
+        c->stream_cap = 0;
+        HIPCHK(c, hipMalloc(&c->stream_pts, npts * 16));
+        c->stream_cap = npts;
+    }
+    if (!c->d_emitters) HIPCHK(c, hipMalloc(&c->d_emitters, 32 * sizeof(float)));
+    if (st->seed_mode == 1u)
+        HIPCHK(c, hipMemcpyAsync(c->d_emitters, emitters, 32 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, launch_streams(c, st, view, c->d_emitters));
+    if (out) HIPCHK(c, hipMemcpyAsync(out, c->stream_pts, npts * 16, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return AERO_OK;
 }

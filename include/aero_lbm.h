@@ -186,6 +186,66 @@ int aero_write_populations(aero_ctx* ctx, const float* in_19n); /* newest buffer
 /* Number of cells where the stability guard (wgsl.ts:238-243) fired since reset. */
 int aero_guard_count(aero_ctx* ctx, uint64_t* count);
 int aero_sync(aero_ctx* ctx);
+/* Overwrites the fields buffer ((ux,uy,uz,rho) per cell; rho <= 0 marks a solid for
+ * the field consumers below), e.g. to sample a synthetic flow. */
+int aero_write_fields(aero_ctx* ctx, const float* in_n4);
+
+/* ---- Field consumers ----
+ * The readers of the fields and the newest temperatures that the reference runs
+ * after every frame's LBM and heat steps (engine.ts:593-642,713-714).  Sampling is
+ * sampleFlow (wgsl.ts:331-355): trilinear over the fluid corners of the
+ * cell-centred grid, renormalised, the z < 0 half mirrored (u_z sign flipped). */
+/* == WGSL struct View (wgsl.ts:431-434), 32 bytes; engine.ts:503-519 fills it. */
+typedef struct { float u_in, v_real, cp_min, cp_max, t_min, t_max, speed_max, vort_max; } aero_view;
+/* == WGSL struct PSim (wgsl.ts:518-523), 64 bytes; engine.ts:595-611 fills it. */
+typedef struct {
+    uint32_t count, spawn_mode, color_mode, frame;
+    float dt, life;
+    uint32_t scatter, rake_rows;
+    float spawn_min[3];
+    uint32_t rake_cols;
+    float spawn_max[3];
+    float pad;
+} aero_psim;
+/* == WGSL struct Stream (render-wgsl.ts:521-526), 64 bytes; engine.ts:633-641 fills it. */
+typedef struct {
+    uint32_t lines, steps, seed_mode, color_mode;
+    float seed_min[3];
+    uint32_t rows;
+    float seed_max[3];
+    uint32_t cols;
+    float step_len, time;
+    uint32_t per_emitter, emitters;
+} aero_stream;
+/* Colour field ids of fieldValue (wgsl.ts:441-457). */
+#define AERO_FIELD_SPEED 0
+#define AERO_FIELD_PRESSURE 1
+#define AERO_FIELD_TEMPERATURE 2
+#define AERO_FIELD_VORTICITY 3
+#define AERO_FIELD_WAKE 4
+
+/* Probes at n points pts[3i..3i+2]: out[8i..8i+7] = ux, uy, uz, rho, temp, fluid
+ * weight (sampleFlow), vorticityAt (wgsl.ts:356-363) and fieldValue(color_mode)
+ * (wgsl.ts:441-457).  No counterpart entry point in the reference, where these are
+ * WGSL functions inside the particle, streamline and render shaders. */
+int aero_sample_flow(aero_ctx* ctx, const float* pts_n3, uint32_t n, const aero_view* view, uint32_t color_mode,
+                     float* out_n8);
+/* The particle buffer (engine.ts:243: count x 32 bytes, zero-filled): per particle
+ * pos (x, y, z, age) then vel (dx, dy, dz, colour value), the WGSL Particle
+ * (wgsl.ts:513-515).  alloc replaces any previous buffer. */
+int aero_particles_alloc(aero_ctx* ctx, uint32_t count);
+/* One PARTICLES_SIM dispatch (wgsl.ts:517-595) over psim->count <= allocated
+ * particles.  spawn_mode 1 (rake) needs rake_rows * rake_cols > 0. */
+int aero_particles_step(aero_ctx* ctx, const aero_psim* psim, const aero_view* view);
+int aero_particles_read(aero_ctx* ctx, float* out_count8);
+int aero_particles_write(aero_ctx* ctx, const float* in_count8);
+/* One STREAM_SIM dispatch (render-wgsl.ts:520-576): lines x steps points
+ * (x, y, z, colour value or -1 once the line has died), line-major, into out
+ * (nullable: the points then stay on the device only).  emitters: 8 x (x, y, z, r),
+ * read for seed_mode 1, which needs 1 <= emitters <= 8 and per_emitter >= 1; the
+ * rake (seed_mode 0) needs rows, cols >= 1. */
+int aero_streamlines(aero_ctx* ctx, const aero_stream* stream, const aero_view* view, const float* emitters_32,
+                     float* out_lines_steps_4);
 
 /* ---- Batched tunnels ----
  * Replaces the reference's sequential ride-height sweep (src/ui/aeromap.ts:73-114,

@@ -186,3 +186,57 @@ class SDFOracle:
         d = self.lib.orc_sdf_point(C.byref(sh), _p(prof), C.cast(arr, C.c_void_p) if arr else None, np_,
                                    x, y, z, C.byref(m))
         return d, m.value
+
+
+class FlowOracle:
+    """f32 restatement of the field consumers (flow_oracle.c): sampleFlow,
+    vorticityAt, fieldValue, PARTICLES_SIM and STREAM_SIM.  Uniform arguments are
+    any ctypes structures with the aero_view / aero_psim / aero_stream layouts."""
+
+    def __init__(self):
+        path = LIBDIR / "libflow_oracle.so"
+        if not path.exists():
+            build()
+        L = self.lib = C.CDLL(str(path))
+        vp = C.c_void_p
+        L.fo_pcg.restype = C.c_uint32
+        L.fo_pcg.argtypes = [C.c_uint32]
+        L.fo_sample_points.argtypes = [C.POINTER(OGrid), vp, vp, vp, C.c_uint32, vp, C.c_uint32, vp]
+        L.fo_particles_step.argtypes = [C.POINTER(OGrid), vp, vp, vp, vp, vp]
+        L.fo_streamlines.argtypes = [C.POINTER(OGrid), vp, vp, vp, vp, vp, vp]
+
+    @staticmethod
+    def _s(st, size):
+        assert C.sizeof(st) == size, (type(st), C.sizeof(st))
+        return C.cast(C.byref(st), C.c_void_p)
+
+    def pcg(self, v: int) -> int:
+        return int(self.lib.fo_pcg(v & 0xFFFFFFFF))
+
+    def sample(self, g: OGrid, fields, temps, view, mode, pts) -> np.ndarray:
+        f = np.ascontiguousarray(fields, dtype=np.float32).reshape(g.n, 4)
+        t = np.ascontiguousarray(temps, dtype=np.float32).reshape(g.n)
+        p = np.ascontiguousarray(pts, dtype=np.float32).reshape(-1, 3)
+        out = np.empty((p.shape[0], 8), np.float32)
+        self.lib.fo_sample_points(C.byref(g), _p(f), _p(t), self._s(view, 32), mode, _p(p), p.shape[0], _p(out))
+        return out
+
+    def particles_step(self, g: OGrid, fields, temps, psim, view, parts) -> np.ndarray:
+        """One dispatch over psim.count particles; returns the updated (count, 8) copy."""
+        f = np.ascontiguousarray(fields, dtype=np.float32).reshape(g.n, 4)
+        t = np.ascontiguousarray(temps, dtype=np.float32).reshape(g.n)
+        out = np.array(parts, dtype=np.float32, copy=True).reshape(-1, 8)
+        assert out.shape[0] >= psim.count
+        self.lib.fo_particles_step(C.byref(g), _p(f), _p(t), self._s(psim, 64), self._s(view, 32), _p(out))
+        return out
+
+    def streamlines(self, g: OGrid, fields, temps, stream, view, emitters=None) -> np.ndarray:
+        f = np.ascontiguousarray(fields, dtype=np.float32).reshape(g.n, 4)
+        t = np.ascontiguousarray(temps, dtype=np.float32).reshape(g.n)
+        em = np.zeros((8, 4), np.float32)
+        if emitters is not None:
+            e = np.asarray(emitters, np.float32).reshape(-1, 4)
+            em[:e.shape[0]] = e
+        out = np.empty((stream.lines, stream.steps, 4), np.float32)
+        self.lib.fo_streamlines(C.byref(g), _p(f), _p(t), self._s(stream, 64), self._s(view, 32), _p(em), _p(out))
+        return out

@@ -45,6 +45,11 @@ def test_struct_layouts():
     assert C.sizeof(_lib.Prim) == 64
     assert C.sizeof(_lib.Rig) == (71 + 4 + 12) * 16
     assert _lib.Shape.p_origin.offset == 48 and _lib.Shape.duct.offset == 92
+    # WGSL View / PSim / Stream (wgsl.ts:431-434,518-523; render-wgsl.ts:521-526)
+    assert C.sizeof(_lib.View) == 32 and C.sizeof(_lib.PSim) == 64 and C.sizeof(_lib.Stream) == 64
+    assert _lib.PSim.spawn_min.offset == 32 and _lib.PSim.rake_cols.offset == 44 and _lib.PSim.spawn_max.offset == 48
+    assert _lib.Stream.seed_min.offset == 16 and _lib.Stream.rows.offset == 28 and _lib.Stream.seed_max.offset == 32
+    assert _lib.Stream.step_len.offset == 48 and _lib.Stream.emitters.offset == 60
 
 
 def test_create_rejects_bad_arguments_before_any_gpu_call():
