@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload f1|cube|large]
                     [--storage f32|f16s|bf16s] [--math fast|exact] [--no-cpu]
+                    [--no-aeromap] [--no-surrogate] [--no-flow]
 
 Workload (default "f1"): the generic F1 body, 320x96x96 half tunnel at 4.5 cm
 cells, FP32 populations, D3Q19 regularized + Smagorinsky, moving floor, sponge
@@ -147,6 +148,41 @@ def surrogate_bench() -> dict:
     }
 
 
+def flow_bench(e, dispatches: int = 200) -> dict:
+    """The field consumers on the benchmark's settled flow, at the reference's sizes
+    (engine.ts:14-28): 140,000 particles per smoke dispatch, 90 rake streamlines and
+    160 vortex-ring streamlines of 360 points.  Particle dispatches are queued back to
+    back and timed over the batch; a streamline dispatch synchronises (its points stay
+    on the device), so its time includes one launch and one synchronisation."""
+    from aerolbm import flow as FL
+    from aerolbm.engine import ShapeParams
+
+    p = ShapeParams()
+    ff = FL.FlowFrame(e, p, "high")
+    for _ in range(5):
+        ff.run(FL.FlowMode.Smoke, FL.ColorField.Speed, 8)
+    e.sync()
+    t0 = time.perf_counter()
+    for _ in range(dispatches):
+        ff.run(FL.FlowMode.Smoke, FL.ColorField.Speed, 8)
+    e.sync()
+    part_us = (time.perf_counter() - t0) / dispatches * 1e6
+    out = {"particles": {"count": ff.count, "us_per_dispatch": round(part_us, 2),
+                         "particles_per_s": float(f"{ff.count / (part_us / 1e6):.4g}")}}
+    view = FL.view_uniform(e.u_lb, 250.0, p)
+    for name, mode in (("streamlines", FL.FlowMode.Streamlines), ("vortices", FL.FlowMode.Vortices)):
+        st, em = FL.stream_uniform(p, mode, FL.ColorField.Speed, 0.0)
+        e.streamlines(st, view, em, copy=False)
+        k = 10
+        t0 = time.perf_counter()
+        for _ in range(k):
+            e.streamlines(st, view, em, copy=False)
+        us = (time.perf_counter() - t0) / k * 1e6
+        out[name] = {"lines": int(st.lines), "points_per_line": int(st.steps), "us_per_dispatch": round(us, 1),
+                     "points_per_s": float(f"{st.lines * st.steps / (us / 1e6):.4g}")}
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -158,6 +194,7 @@ def main() -> None:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-aeromap", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--no-flow", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -256,6 +293,9 @@ def main() -> None:
         },
         "forces": {"steps": nsteps, "rest_fx": d[0], "front_fy": d[4], "rear_fy": d[7], "guard_hits": guard},
     }
+    if not args.no_flow:
+        print("[bench] field consumers (particles, streamlines)", file=sys.stderr, flush=True)
+        result["field_consumers"] = flow_bench(e)
     e.close()
     if not args.no_aeromap:
         result["aeromap"] = aeromap_bench()

@@ -422,14 +422,15 @@ class HipAeroEngine:
         a = np.ascontiguousarray(p, dtype=np.float32).reshape(getattr(self, "particle_count", 0), 8)
         self._c(self._lib.aero_particles_write(self.ctx, ptr(a)), "aero_particles_write")
 
-    def streamlines(self, stream: "_lib.Stream", view: "_lib.View", emitters=None) -> np.ndarray:
+    def streamlines(self, stream: "_lib.Stream", view: "_lib.View", emitters=None, copy: bool = True):
         """One STREAM_SIM dispatch (render-wgsl.ts:551-575): (lines, steps, 4) points,
-        the fourth component the colour value or -1 after the line died."""
+        the fourth component the colour value or -1 after the line died (None with
+        copy=False: the points stay on the device)."""
         em = np.zeros((8, 4), np.float32)
         if emitters is not None:
             e = np.asarray(emitters, np.float32).reshape(-1, 4)
             em[:e.shape[0]] = e
-        out = np.empty((stream.lines, stream.steps, 4), np.float32)
-        self._c(self._lib.aero_streamlines(self.ctx, C.byref(stream), C.byref(view), ptr(em), ptr(out)),
-                "aero_streamlines")
+        out = np.empty((stream.lines, stream.steps, 4), np.float32) if copy else None
+        self._c(self._lib.aero_streamlines(self.ctx, C.byref(stream), C.byref(view), ptr(em),
+                                           ptr(out) if copy else None), "aero_streamlines")
         return out

@@ -37,6 +37,7 @@ static napi_value throw_rc(napi_env env, int rc, aero_ctx* c, const char* what) 
 typedef struct {
     aero_ctx* c;
     uint32_t n;
+    uint32_t nparts;  /* particles allocated by particlesAlloc */
 } handle_t;
 
 static void finalize_handle(napi_env env, void* data, void* hint) {
@@ -132,6 +133,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     }
     h->c = ctx;
     h->n = grid.n;
+    h->nparts = 0;
     napi_value ext;
     if (napi_create_external(env, h, finalize_handle, NULL, &ext) != napi_ok) {
         aero_destroy(ctx);
@@ -380,6 +382,171 @@ static napi_value js_sync(napi_env env, napi_callback_info info) {
     return NULL;
 }
 
+/* ---- field consumers (wgsl.ts:324-457,517-595; render-wgsl.ts:520-576) ---- */
+
+static int get_view(napi_env env, napi_value v, aero_view* out) {
+    size_t n = 0;
+    const float* f = (const float*)typed(env, v, napi_float32_array, &n);
+    if (!f) return 0;
+    if (n != 8) {
+        napi_throw_range_error(env, "AERO_ARGS", "view needs 8 values (the View uniform)");
+        return 0;
+    }
+    memcpy(out, f, sizeof *out);
+    return 1;
+}
+
+/* a packed 64-byte uniform (PSim or Stream) */
+static int get_u64b(napi_env env, napi_value v, void* out) {
+    size_t n = 0;
+    const uint8_t* b = (const uint8_t*)typed(env, v, napi_uint8_array, &n);
+    if (!b) return 0;
+    if (n != 64) {
+        napi_throw_range_error(env, "AERO_ARGS", "uniform must be 64 bytes");
+        return 0;
+    }
+    memcpy(out, b, 64);
+    return 1;
+}
+
+/* writeFields(ctx, fields: Float32Array[4n]) */
+static napi_value js_write_fields(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!get_args(env, info, 2, a)) return NULL;
+    handle_t* h = get_handle(env, a[0]);
+    if (!h) return NULL;
+    size_t n = 0;
+    const float* f = (const float*)typed(env, a[1], napi_float32_array, &n);
+    if (!f) return NULL;
+    if (n != (size_t)h->n * 4) {
+        napi_throw_range_error(env, "AERO_ARGS", "fields must hold 4 values per cell");
+        return NULL;
+    }
+    const int rc = aero_write_fields(h->c, f);
+    if (rc) return throw_rc(env, rc, h->c, "aero_write_fields");
+    return NULL;
+}
+
+/* sampleFlow(ctx, pts: Float32Array[3k], view: Float32Array[8], colorMode) -> Float32Array[8k] */
+static napi_value js_sample_flow(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    if (!get_args(env, info, 4, a)) return NULL;
+    aero_ctx* c = get_ctx(env, a[0]);
+    if (!c) return NULL;
+    size_t n = 0;
+    const float* pts = (const float*)typed(env, a[1], napi_float32_array, &n);
+    if (!pts) return NULL;
+    if (n % 3 || n / 3 > (1u << 26)) {
+        napi_throw_range_error(env, "AERO_ARGS", "points must be whole (x, y, z) triples, at most 2^26");
+        return NULL;
+    }
+    aero_view view;
+    if (!get_view(env, a[2], &view)) return NULL;
+    uint32_t mode = 0;
+    NAPI_CALL(env, napi_get_value_uint32(env, a[3], &mode));
+    void* p = NULL;
+    napi_value arr = make_typed(env, napi_float32_array, n / 3 * 8, 4, &p);
+    if (!arr) return NULL;
+    const int rc = aero_sample_flow(c, pts, (uint32_t)(n / 3), &view, mode, (float*)p);
+    if (rc) return throw_rc(env, rc, c, "aero_sample_flow");
+    return arr;
+}
+
+/* particlesAlloc(ctx, count) */
+static napi_value js_particles_alloc(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!get_args(env, info, 2, a)) return NULL;
+    handle_t* h = get_handle(env, a[0]);
+    if (!h) return NULL;
+    uint32_t count = 0;
+    NAPI_CALL(env, napi_get_value_uint32(env, a[1], &count));
+    h->nparts = 0;
+    const int rc = aero_particles_alloc(h->c, count);
+    if (rc) return throw_rc(env, rc, h->c, "aero_particles_alloc");
+    h->nparts = count;
+    return NULL;
+}
+
+/* particlesStep(ctx, psim: Uint8Array[64], view: Float32Array[8]) */
+static napi_value js_particles_step(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    if (!get_args(env, info, 3, a)) return NULL;
+    aero_ctx* c = get_ctx(env, a[0]);
+    if (!c) return NULL;
+    aero_psim ps;
+    aero_view view;
+    if (!get_u64b(env, a[1], &ps) || !get_view(env, a[2], &view)) return NULL;
+    const int rc = aero_particles_step(c, &ps, &view);
+    if (rc) return throw_rc(env, rc, c, "aero_particles_step");
+    return NULL;
+}
+
+/* readParticles(ctx) -> Float32Array[8 * count]; writeParticles(ctx, Float32Array[8 * count]) */
+static napi_value js_read_particles(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    if (!get_args(env, info, 1, a)) return NULL;
+    handle_t* h = get_handle(env, a[0]);
+    if (!h) return NULL;
+    void* p = NULL;
+    napi_value arr = make_typed(env, napi_float32_array, (size_t)h->nparts * 8, 4, &p);
+    if (!arr) return NULL;
+    if (h->nparts) {
+        const int rc = aero_particles_read(h->c, (float*)p);
+        if (rc) return throw_rc(env, rc, h->c, "aero_particles_read");
+    }
+    return arr;
+}
+
+static napi_value js_write_particles(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!get_args(env, info, 2, a)) return NULL;
+    handle_t* h = get_handle(env, a[0]);
+    if (!h) return NULL;
+    size_t n = 0;
+    const float* f = (const float*)typed(env, a[1], napi_float32_array, &n);
+    if (!f) return NULL;
+    if (n != (size_t)h->nparts * 8) {
+        napi_throw_range_error(env, "AERO_ARGS", "particles must hold 8 values per allocated particle");
+        return NULL;
+    }
+    const int rc = aero_particles_write(h->c, f);
+    if (rc) return throw_rc(env, rc, h->c, "aero_particles_write");
+    return NULL;
+}
+
+/* streamlines(ctx, stream: Uint8Array[64], view: Float32Array[8], emitters: Float32Array[32]|null)
+ *   -> Float32Array[4 * lines * steps] */
+static napi_value js_streamlines(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    if (!get_args(env, info, 4, a)) return NULL;
+    aero_ctx* c = get_ctx(env, a[0]);
+    if (!c) return NULL;
+    aero_stream st;
+    aero_view view;
+    if (!get_u64b(env, a[1], &st) || !get_view(env, a[2], &view)) return NULL;
+    const float* em = NULL;
+    if (!is_nullish(env, a[3])) {
+        size_t n = 0;
+        em = (const float*)typed(env, a[3], napi_float32_array, &n);
+        if (!em) return NULL;
+        if (n != 32) {
+            napi_throw_range_error(env, "AERO_ARGS", "emitters must hold 8 x (x, y, z, r)");
+            return NULL;
+        }
+    }
+    const uint64_t npts = (uint64_t)st.lines * st.steps;
+    if (npts > (1ull << 26)) {
+        napi_throw_range_error(env, "AERO_ARGS", "more than 2^26 streamline points");
+        return NULL;
+    }
+    void* p = NULL;
+    napi_value arr = make_typed(env, napi_float32_array, (size_t)npts * 4, 4, &p);
+    if (!arr) return NULL;
+    const int rc = aero_streamlines(c, &st, &view, em, npts ? (float*)p : NULL);
+    if (rc) return throw_rc(env, rc, c, "aero_streamlines");
+    return arr;
+}
+
 static napi_value js_version(napi_env env, napi_callback_info info) {
     (void)info;
     napi_value v;
@@ -399,7 +566,10 @@ static napi_value init_module(napi_env env, napi_value exports) {
         {"readFields", js_read_fields}, {"readTemps", js_read_temps},
         {"readFlags", js_read_flags},   {"readPopulations", js_read_populations},
         {"guardCount", js_guard_count}, {"sync", js_sync},
-        {"version", js_version},
+        {"version", js_version},        {"writeFields", js_write_fields},
+        {"sampleFlow", js_sample_flow}, {"particlesAlloc", js_particles_alloc},
+        {"particlesStep", js_particles_step}, {"readParticles", js_read_particles},
+        {"writeParticles", js_write_particles}, {"streamlines", js_streamlines},
     };
     for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
         napi_value f;

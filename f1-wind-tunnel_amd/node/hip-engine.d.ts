@@ -7,6 +7,12 @@ export declare const ShapeKind: {
   readonly W17: 5; readonly Empty: 6; readonly Scene: 16;
 };
 export declare const Storage: { readonly F32: 0; readonly F16S: 1; readonly BF16S: 2; readonly MATH_EXACT: 0x100 };
+/** FlowMode (engine.ts:34) and the fieldValue colour ids (wgsl.ts:441-457). */
+export declare const FlowMode: { readonly None: 0; readonly Streamlines: 1; readonly Smoke: 2; readonly Fog: 3; readonly Vortices: 4 };
+export declare const ColorField: { readonly Speed: 0; readonly Pressure: 1; readonly Temperature: 2; readonly Vorticity: 3; readonly Wake: 4 };
+export declare const PARTICLES: { readonly low: number; readonly medium: number; readonly high: number };
+export declare const MAX_LINES: number;
+export declare const STREAM_STEPS: number;
 
 export interface ShapeParams {
   kind: number;
@@ -24,6 +30,20 @@ export interface GridSpec { dims: [number, number, number]; h: number; origin?: 
 export interface EngineOptions { uLb?: number; tau0?: number; smag?: number; storage?: number; kmh?: number }
 
 export declare function packShape(p: ShapeParams, profileMeta?: { originX: number; originY: number; cell: number }): Uint8Array;
+
+/** PSim (wgsl.ts:518-523) and Stream (render-wgsl.ts:521-526) uniforms. */
+export interface PSim {
+  count: number; spawnMode: number; colorMode: number; frame: number; dt: number; life: number; scatter: boolean;
+  rakeRows: number; rakeCols: number; spawnMin: [number, number, number]; spawnMax: [number, number, number];
+}
+export interface StreamConfig {
+  lines: number; steps: number; seedMode: number; colorMode: number; min: [number, number, number]; rows: number;
+  max: [number, number, number]; cols: number; stepLen: number; time: number; perEmitter: number; emitters: number;
+}
+export declare function packPSim(ps: PSim): Uint8Array;
+export declare function packStream(st: StreamConfig): Uint8Array;
+/** viewUniform (engine.ts:503-519): uIn, vReal, cpMin, cpMax, tMin, tMax, speedMax, vortMax. */
+export declare function viewUniform(uLb: number, kmh: number, p?: ShapeParams): Float32Array;
 
 export declare class HipAeroEngine {
   constructor(quality?: "low" | "medium" | "high" | GridSpec, opts?: EngineOptions);
@@ -46,4 +66,16 @@ export declare class HipAeroEngine {
   populations(): Float32Array;
   guardCount(): number;
   sync(): void;
+  /** One frame's flow dispatch (engine.ts:588-642): particles for Smoke/Fog (returns null),
+   *  lines x 360 points of (x, y, z, value or -1) for Streamlines/Vortices. */
+  flowFrame(flowMode: number, colorField: number, steps: number, dt?: number): Float32Array | null;
+  spawnRegion(): { min: [number, number, number]; max: [number, number, number]; rows: number; cols: number };
+  emitterPoints(): { e: Float32Array; count: number };
+  allocParticles(count: number): void;
+  /** 8 floats per particle: pos x, y, z, age, step dx, dy, dz, colour value. */
+  particles(): Float32Array;
+  writeParticles(p: Float32Array): void;
+  /** 8 floats per point: ux, uy, uz, rho, temp, fluid weight, vorticity, field value. */
+  sampleFlow(points: Float32Array, view: Float32Array, colorMode?: number): Float32Array;
+  writeFields(f: Float32Array): void;
 }

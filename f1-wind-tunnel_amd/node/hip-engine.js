@@ -6,7 +6,8 @@
 //
 // Constants and packing follow engine.ts: quality presets (:14-28), domain length and
 // origin (:110, :249, :263), lattice constants, the Shape uniform (:429-448), the
-// frame's evenSteps LBM + 2 heat steps (:585, :708-712), force scaling (:739-761).
+// frame's evenSteps LBM + 2 heat steps (:585, :708-712), force scaling (:739-761), and
+// the flow visualisation uniforms and dispatches (:503-572, :588-642, :731-737).
 
 const path = require("path");
 
@@ -28,6 +29,67 @@ const COOLING_HEAT = [1.55, 1, 0.6];
 const COOLING_DUCT = [0, 1, 1.3];
 
 const ShapeKind = { F1: 0, Sphere: 1, Cube: 2, Wing: 3, Custom: 4, W17: 5, Empty: 6, Scene: 16 };
+const FlowMode = { None: 0, Streamlines: 1, Smoke: 2, Fog: 3, Vortices: 4 };
+const ColorField = { Speed: 0, Pressure: 1, Temperature: 2, Vorticity: 3, Wake: 4 };
+const PARTICLES = { low: 40000, medium: 80000, high: 140000 };
+const MAX_LINES = 160;
+const STREAM_STEPS = 360;
+const isRaceCar = (k) => k === ShapeKind.F1 || k === ShapeKind.W17;
+
+/** shapeBounds (engine.ts:474-488) for the built-in kinds. */
+function shapeBounds(p) {
+  switch (p.kind) {
+    case ShapeKind.F1: return { min: [-0.08, 0, -0.98], max: [5.46, p.rideH + 1.34, 0.98] };
+    case ShapeKind.W17: return { min: [-0.3, 0, -0.98], max: [5.2, p.rideH + 1.16, 0.98] };
+    case ShapeKind.Empty: return { min: [2.6, 0.5, -0.05], max: [2.7, 0.6, 0.05] };
+    case ShapeKind.Sphere: return { min: [1.95, 0.35, -0.65], max: [3.25, 1.65, 0.65] };
+    case ShapeKind.Cube: return { min: [2.05, 0.45, -0.55], max: [3.15, 1.55, 0.55] };
+    case ShapeKind.Wing: return { min: [1.3, 0.2, -1.3], max: [3.6, 2.2, 1.3] };
+    default: {
+      const zr = p.customWidth === undefined ? 0.6 : p.customWidth;
+      return { min: [-0.05, 0, -zr - 0.05], max: [1.05, 1.05, zr + 0.05] };
+    }
+  }
+}
+
+/** w17Place (engine.ts:136-141; CAR_LAYOUT of car-model.ts:15-20): a body-frame point
+ *  moved with ride height and rake. */
+function w17Place(p, x, y, z) {
+  const rake = Math.atan((p.rearRideH - p.rideH) / (4.4 - 1.0));
+  const c = Math.cos(rake), s = Math.sin(rake);
+  const bx = x - 1.0, by = y - 0.092;
+  return [1.0 + c * bx - s * by, p.rideH + s * bx + c * by, z];
+}
+
+/** viewUniform (engine.ts:503-519) as the 8-float View. */
+function viewUniform(uLb, kmh, p) {
+  const vReal = kmh / 3.6;
+  const stag = 0.5 * vReal * vReal / 1005;
+  const hot = !!p && isRaceCar(p.kind) && !!(p.heatBrakes || p.heatEngine || p.heatRad);
+  return new Float32Array([uLb, vReal, -1.1, 0.7, hot ? 0 : -stag * 1.4, hot ? 220 : stag * 1.05, 1.5, 0.6]);
+}
+
+/** PSim (wgsl.ts:518-523) packed to 64 bytes. */
+function packPSim(ps) {
+  const buf = new ArrayBuffer(64);
+  const u = new Uint32Array(buf), f = new Float32Array(buf);
+  u[0] = ps.count; u[1] = ps.spawnMode; u[2] = ps.colorMode; u[3] = ps.frame;
+  f[4] = ps.dt; f[5] = ps.life; u[6] = ps.scatter ? 1 : 0; u[7] = ps.rakeRows;
+  f[8] = ps.spawnMin[0]; f[9] = ps.spawnMin[1]; f[10] = ps.spawnMin[2]; u[11] = ps.rakeCols;
+  f[12] = ps.spawnMax[0]; f[13] = ps.spawnMax[1]; f[14] = ps.spawnMax[2]; f[15] = 0;
+  return new Uint8Array(buf);
+}
+
+/** Stream (render-wgsl.ts:521-526) packed to 64 bytes. */
+function packStream(st) {
+  const buf = new ArrayBuffer(64);
+  const u = new Uint32Array(buf), f = new Float32Array(buf);
+  u[0] = st.lines; u[1] = st.steps; u[2] = st.seedMode; u[3] = st.colorMode;
+  f[4] = st.min[0]; f[5] = st.min[1]; f[6] = st.min[2]; u[7] = st.rows;
+  f[8] = st.max[0]; f[9] = st.max[1]; f[10] = st.max[2]; u[11] = st.cols;
+  f[12] = st.stepLen; f[13] = st.time; u[14] = st.perEmitter; u[15] = st.emitters;
+  return new Uint8Array(buf);
+}
 const Storage = { F32: 0, F16S: 1, BF16S: 2, MATH_EXACT: 0x100 };
 
 // The app's initial setup (src/main.ts:99-100) fills fields a caller leaves out.
@@ -99,6 +161,12 @@ class HipAeroEngine {
     this.kmh = opts.kmh !== undefined ? opts.kmh : 250;
     this.frameNo = 0;
     this.stepsSinceRead = 0;
+    this.quality = typeof quality === "string" ? quality : "high";
+    this.shapeParams = Object.assign({}, SHAPE_DEFAULTS);
+    this.flowTime = 0;
+    this.flowFrameNo = 0;
+    this.scatterFrames = 2;
+    this.particleCount = 0;
     this.onForces = undefined;
     this.profileMeta = undefined;
     this.profile = null;
@@ -128,6 +196,7 @@ class HipAeroEngine {
 
   /** engine.ts:426-470; prims (Uint8Array of 64-byte aero_prim records) for kind 16 scenes */
   setShape(p, prims) {
+    this.shapeParams = Object.assign({}, SHAPE_DEFAULTS, p);
     const needsProfile = p.kind === ShapeKind.Custom;
     let prof = null;
     if (needsProfile) prof = this.profile || new Float32Array(PROFILE_W * PROFILE_H).fill(10);
@@ -180,9 +249,101 @@ class HipAeroEngine {
     return { fields: addon.readFields(this.ctx), solid: flags.solid, temps: addon.readTemps(this.ctx) };
   }
 
+  // ---- flow visualisation (engine.ts:526-572, :588-642, :731-737) ----
+
+  /** spawnRegion (engine.ts:731-737) */
+  spawnRegion() {
+    const p = this.shapeParams, b = shapeBounds(p);
+    const zMax = Math.max(0.3, b.max[2] + 0.25);
+    const yMax = Math.max(0.5, b.max[1] + 0.25);
+    const yMin = isRaceCar(p.kind) ? 0.04 : Math.max(0.04, b.min[1] - 0.25);
+    return { min: [ORIGIN_X + this.h * 2, yMin, 0.0], max: [0, yMax, zMax], rows: 12, cols: 7 };
+  }
+
+  /** emitterPoints (engine.ts:526-556) */
+  emitterPoints() {
+    const p = this.shapeParams;
+    let pts;
+    if (p.kind === ShapeKind.W17) {
+      const place = (x, y, z, r) => [...w17Place(p, x, y, z), r];
+      pts = [
+        ...(p.frontWingOn !== false ? [place(0.62, 0.44, 0.69, 0.05), place(0.5, 0.27, 0.2, 0.04)] : []),
+        ...(p.rearWingOn !== false ? [place(5.13, 0.96, 0.57, 0.05)] : []),
+        place(2.4, 0.16, 0.86, 0.05),
+      ];
+    } else if (isRaceCar(p.kind)) {
+      const yw = 0.19 + p.rideH * 0.45, yr = p.rideH + 0.8;
+      pts = [[0.52, yw + 0.2, 0.926, 0.06], [0.52, yw + 0.07, 0.21, 0.05], [5.36, yr + 0.26, 0.494, 0.06],
+             [2.22, p.rideH + 0.06, 0.79, 0.05]];
+    } else {
+      const b = shapeBounds(p);
+      pts = [[b.max[0], b.max[1] - 0.05, b.max[2] - 0.05, 0.08], [b.max[0], b.min[1] + 0.05, b.max[2] - 0.05, 0.08],
+             [b.max[0], (b.min[1] + b.max[1]) / 2, 0.05, 0.1]];
+    }
+    const e = new Float32Array(32);
+    pts.forEach((q, i) => e.set(q, i * 4));
+    return { e, count: pts.length };
+  }
+
+  /** streamConfig (engine.ts:558-572) */
+  streamConfig(flowMode) {
+    if (flowMode === FlowMode.Vortices) {
+      const em = this.emitterPoints();
+      const perEmitter = Math.floor(MAX_LINES / em.count);
+      return { lines: perEmitter * em.count, seedMode: 1, perEmitter, emitters: em.count, e: em.e, stepLen: 0.022,
+               rows: 1, cols: 1, min: [0, 0, 0], max: [0, 0, 0] };
+    }
+    const b = shapeBounds(this.shapeParams);
+    const f1 = isRaceCar(this.shapeParams.kind);
+    const rows = 18, cols = 5;
+    return { lines: rows * cols, seedMode: 0, perEmitter: 1, emitters: 1, e: this.emitterPoints().e, stepLen: 0.03,
+             rows, cols, min: [-1.2, f1 ? 0.05 : Math.max(0.04, b.min[1] - 0.2), 0.03],
+             max: [0, f1 ? 1.3 : b.max[1] + 0.2, f1 ? 0.95 : b.max[2] + 0.2] };
+  }
+
+  /** The flow dispatch of one frame (engine.ts:588-642, :713-714) after `steps` LBM
+   *  steps: particles for Smoke / Fog, streamline points (returned) otherwise. */
+  flowFrame(flowMode, colorField, steps, dt) {
+    const rate = Math.min(1.6, Math.max(0.35, this.kmh / 250));
+    this.flowFrameNo++;
+    this.flowTime += (dt === undefined ? 1 / 60 : dt) * rate;
+    const view = viewUniform(this.uLb, this.kmh, this.shapeParams);
+    const colorMode = colorField === ColorField.Vorticity ? 0 : colorField;
+    const even = Math.max(2, steps - (steps % 2));
+    if (flowMode === FlowMode.Smoke || flowMode === FlowMode.Fog) {
+      const count = PARTICLES[this.quality] || PARTICLES.high;
+      if (this.particleCount !== count) this.allocParticles(count);
+      const sp = this.spawnRegion();
+      addon.particlesStep(this.ctx, packPSim({
+        count, spawnMode: flowMode === FlowMode.Fog ? 0 : 1, colorMode, frame: this.flowFrameNo, dt: even * rate,
+        life: 900, scatter: this.scatterFrames > 0, rakeRows: sp.rows, rakeCols: sp.cols, spawnMin: sp.min,
+        spawnMax: sp.max }), view);
+      if (this.scatterFrames > 0) this.scatterFrames--;
+      return null;
+    }
+    if (flowMode === FlowMode.Streamlines || flowMode === FlowMode.Vortices) {
+      const sc = this.streamConfig(flowMode);
+      return addon.streamlines(this.ctx, packStream(Object.assign({}, sc, { steps: STREAM_STEPS, colorMode,
+                                                                            time: this.flowTime })), view, sc.e);
+    }
+    return null;
+  }
+
+  /** The zero-filled particle buffer (engine.ts:243), 8 floats per particle. */
+  allocParticles(count) {
+    addon.particlesAlloc(this.ctx, count);
+    this.particleCount = count;
+  }
+  particles() { return addon.readParticles(this.ctx); }
+  writeParticles(p) { addon.writeParticles(this.ctx, p); }
+  /** Probes: 8 floats per point (u, rho, temp, fluid weight, vorticity, field value). */
+  sampleFlow(points, view, colorMode) { return addon.sampleFlow(this.ctx, points, view, colorMode || 0); }
+  writeFields(f) { addon.writeFields(this.ctx, f); }
+
   populations() { return addon.readPopulations(this.ctx); }
   guardCount() { return addon.guardCount(this.ctx); }
   sync() { addon.sync(this.ctx); }
 }
 
-module.exports = { HipAeroEngine, ShapeKind, Storage, packShape, addon, QUALITIES, LATTICE_U, TAU0, SMAG };
+module.exports = { HipAeroEngine, ShapeKind, Storage, FlowMode, ColorField, packShape, packPSim, packStream, viewUniform,
+                   addon, QUALITIES, PARTICLES, MAX_LINES, STREAM_STEPS, LATTICE_U, TAU0, SMAG };

@@ -3,7 +3,7 @@
 // GPU): closed-form voxel count of the 1 m cube, the uniform-flow fixed point of an
 // empty tunnel, force scaling and readback, error mapping.  Prints one JSON line with
 // the figures a caller can cross-check against the Python host.
-const { HipAeroEngine, ShapeKind, Storage } = require("./hip-engine");
+const { HipAeroEngine, ShapeKind, Storage, FlowMode, ColorField } = require("./hip-engine");
 
 const f32 = (x) => Math.fround(x);
 const cubeGrid = { dims: [128, 64, 64], h: f32(1 / 22), origin: [f32(2.6 - 42 / 22), 0, 0] };
@@ -33,9 +33,29 @@ const check = (name, ok, detail) => {
   const raw = require("./hip-engine").addon.readForces(e.ctx, false);
   check("reset_zeroes", raw.steps === 0, raw.steps);
   out.cube_forces = fd;
+
+  // 3) flow visualisation on the settled cube flow: three smoke frames, a pressure-
+  // coloured fog frame, a streamline and a vortex frame; sums for the Python cross-check
+  // checksum: the float32 words summed as unsigned integers modulo 2^32
+  const sum = (a) => {
+    const w = new Uint32Array(a.buffer, a.byteOffset, a.length);
+    let s = 0;
+    for (let i = 0; i < w.length; i++) s = (s + w[i]) >>> 0;
+    return s;
+  };
+  for (let k = 0; k < 3; k++) e.flowFrame(FlowMode.Smoke, ColorField.Speed, 8);
+  e.flowFrame(FlowMode.Fog, ColorField.Pressure, 8);
+  const parts = e.particles();
+  const lines = e.flowFrame(FlowMode.Streamlines, ColorField.Speed, 8);
+  const vort = e.flowFrame(FlowMode.Vortices, ColorField.Wake, 8);
+  let live = 0;
+  for (let i = 3; i < lines.length; i += 4) if (lines[i] >= 0) live++;
+  check("particles", parts.length === 140000 * 8 && parts.some((v, i) => i % 8 === 3 && v > 0), parts.length);
+  check("streamlines", lines.length === 90 * 360 * 4 && live > 90 * 10, live);
+  out.flow = { particles: sum(parts), lines: sum(lines), vortices: sum(vort), live };
   e.destroy();
 
-  // 3) an empty tunnel at uniform inflow is a fixed point of the whole step
+  // 4) an empty tunnel at uniform inflow is a fixed point of the whole step
   const g = new HipAeroEngine({ dims: [40, 16, 12], h: 0.1, origin: [0, 0, 0] },
                               { uLb: 0.05, tau0: 0.51, smag: 0.1, storage: Storage.F32 | Storage.MATH_EXACT });
   g.setShape({ kind: ShapeKind.Empty, rideH: 0.1, rearRideH: 0.12 });
@@ -55,7 +75,7 @@ const check = (name, ok, detail) => {
   check("frame_forces", got !== null && Math.abs(got.dragN) < 1e-3, got && got.dragN);
   g.destroy();
 
-  // 4) errors surface as thrown Errors with the library's message
+  // 5) errors surface as thrown Errors with the library's message
   let msg = "";
   try {
     new HipAeroEngine({ dims: [16, 8, 8], h: 0.1 }, { tau0: 0.5 });

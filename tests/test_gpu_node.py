@@ -39,3 +39,21 @@ def test_node_engine_checks_and_python_agreement():
     js = out["cube_forces"]
     assert f.dragN == pytest.approx(js["dragN"], rel=1e-12)
     assert f.downforceN == pytest.approx(js["downforceN"], rel=1e-12, abs=1e-12)
+    # the same flow frames through the Python host (aerolbm.flow.FlowFrame): identical
+    # particle and streamline buffers
+    from aerolbm import flow as FL
+    p = aerolbm.ShapeParams(kind=aerolbm.ShapeKind.Cube, rideH=0.1, rearRideH=0.12)
+    ff = FL.FlowFrame(e, p, "high")
+    for _ in range(3):
+        ff.run(FL.FlowMode.Smoke, FL.ColorField.Speed, 8)
+    ff.run(FL.FlowMode.Fog, FL.ColorField.Pressure, 8)
+    parts = e.particles()
+    lines = ff.run(FL.FlowMode.Streamlines, FL.ColorField.Speed, 8)
+    vort = ff.run(FL.FlowMode.Vortices, FL.ColorField.Wake, 8)
+    fl = out["flow"]
+
+    def words(a):  # the float32 words summed as unsigned integers modulo 2^32
+        return int(np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64).sum() % (1 << 32))
+    assert words(parts) == fl["particles"]
+    assert words(lines) == fl["lines"]
+    assert words(vort) == fl["vortices"]

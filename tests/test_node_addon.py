@@ -57,3 +57,36 @@ console.log(Buffer.from(b).toString('hex'));
     p = aerolbm.ShapeParams(**{k: v for k, v in params.items()})
     py = bytes(aerolbm.shape_uniform(p))
     assert bytes.fromhex(js) == py
+
+
+@pytest.mark.parametrize("kind,kmh,mode", [(0, 250.0, 2), (2, 180.0, 3), (5, 320.0, 1), (1, 90.0, 4), (5, 250.0, 4),
+                                           (0, 300.0, 4)])
+def test_flow_uniforms_match_python_host(kind, kmh, mode):
+    """viewUniform, the frame's PSim and streamConfig packed by the JavaScript host are
+    byte-identical to the Python host's (engine.ts:503-572,595-641,731-737)."""
+    from aerolbm import flow as FL
+    p = aerolbm.ShapeParams(kind=kind, rideH=0.11, rearRideH=0.15)
+    h = 0.045
+    js = json.loads(node(f"""
+const m = require('./hip-engine');
+const self = {{ shapeParams: Object.assign({{}}, {{kind: {kind}, rideH: 0.11, rearRideH: 0.15, heatBrakes: true,
+               heatEngine: true, heatRad: true, customWidth: 0.6, frontWingOn: true, rearWingOn: true}}), h: {h} }};
+self.emitterPoints = m.HipAeroEngine.prototype.emitterPoints;
+const sp = m.HipAeroEngine.prototype.spawnRegion.call(self);
+const view = m.viewUniform(0.0925, {kmh}, self.shapeParams);
+const rate = Math.min(1.6, Math.max(0.35, {kmh} / 250));
+const ps = m.packPSim({{count: 140000, spawnMode: {mode} === 3 ? 0 : 1, colorMode: 0, frame: 7, dt: 8 * rate, life: 900,
+                       scatter: true, rakeRows: sp.rows, rakeCols: sp.cols, spawnMin: sp.min, spawnMax: sp.max}});
+const sc = m.HipAeroEngine.prototype.streamConfig.call(self, {mode} === 4 ? 4 : 1);
+const st = m.packStream(Object.assign({{}}, sc, {{steps: m.STREAM_STEPS, colorMode: 0, time: 0.5}}));
+console.log(JSON.stringify({{view: Buffer.from(view.buffer).toString('hex'), psim: Buffer.from(ps).toString('hex'),
+                             stream: Buffer.from(st).toString('hex'), em: Buffer.from(sc.e.buffer).toString('hex')}}));
+"""))
+    view = FL.view_uniform(0.0925, kmh, p)
+    assert bytes.fromhex(js["view"]) == bytes(view)
+    ps = FL.psim_uniform(140000, FL.FlowMode.Fog if mode == 3 else FL.FlowMode.Smoke, 0, 7, 8, h, p, kmh,
+                         scatter=True)
+    assert bytes.fromhex(js["psim"]) == bytes(ps)
+    st, em = FL.stream_uniform(p, FL.FlowMode.Vortices if mode == 4 else FL.FlowMode.Streamlines, 0, 0.5)
+    assert bytes.fromhex(js["stream"]) == bytes(st)
+    assert bytes.fromhex(js["em"]) == em.tobytes()
