@@ -131,6 +131,25 @@ def batched(points: Sequence[Workload], settle: int, measure: int, storage: str 
     return MapRun(out, wall, ms, storage, b.cells, settle + measure, shape)
 
 
+# The reference's sweep (src/ui/aeromap.ts:15-19): ride heights in metres.
+REFERENCE_FRONT = (0.08, 0.11, 0.14, 0.17, 0.2)
+REFERENCE_REAR = (0.1, 0.14, 0.18, 0.22, 0.26)
+KGF = 9.81
+
+
+def map_cells(forces, pairs, balance=None) -> List[dict]:
+    """The cells of src/ui/aeromap.ts:97-101 from one Forces per (front, rear) pair:
+    down = downforce / 9.81 (kg), eff = downforce / max(drag, 1), bal = the front-axle
+    share of the wing load.  The reference takes bal from its panel-method wing loads on
+    the W17 only and uses 0.5 otherwise; `balance` (one value per pair) overrides it."""
+    out = []
+    for k, (F, (f, r)) in enumerate(zip(forces, pairs)):
+        bal = 0.5 if balance is None or balance[k] is None else float(balance[k])
+        out.append({"front": f, "rear": r, "down": F.downforceN / KGF,
+                    "eff": F.downforceN / max(F.dragN, 1.0), "bal": bal})
+    return out
+
+
 def force_errors(run: MapRun, ref: MapRun) -> np.ndarray:
     """Per point: |dF| / |F_ref| with F = (<Fx>, <Fy>), the error measure of the
     map's two coefficients that stays defined when one of them is near zero."""

@@ -38,7 +38,21 @@ typedef struct {
     aero_ctx* c;
     uint32_t n;
     uint32_t nparts;  /* particles allocated by particlesAlloc */
+    int owned;        /* 0: an instance of a batch, released with the batch */
 } handle_t;
+
+/* A batch handle: the batch and one instance handle per tunnel.  batchDestroy clears
+ * the instance handles' contexts, so later calls on them throw.  The batch and its
+ * instance handles are freed together, when the garbage collector has finalised the
+ * batch external and every instance external made by batchInstances (refs counts
+ * them). */
+typedef struct {
+    aero_batch* b;
+    uint32_t n;
+    handle_t** inst;
+    uint32_t refs;
+    int wrapped;
+} batch_t;
 
 static void finalize_handle(napi_env env, void* data, void* hint) {
     (void)env;
@@ -134,6 +148,7 @@ static napi_value js_create(napi_env env, napi_callback_info info) {
     h->c = ctx;
     h->n = grid.n;
     h->nparts = 0;
+    h->owned = 1;
     napi_value ext;
     if (napi_create_external(env, h, finalize_handle, NULL, &ext) != napi_ok) {
         aero_destroy(ctx);
@@ -149,6 +164,10 @@ static napi_value js_destroy(napi_env env, napi_callback_info info) {
     if (!get_args(env, info, 1, a)) return NULL;
     handle_t* h = get_handle(env, a[0]);
     if (!h) return NULL;
+    if (!h->owned) {
+        napi_throw_error(env, "AERO_ARGS", "a batch instance is released with its batch (batchDestroy)");
+        return NULL;
+    }
     const int rc = aero_destroy(h->c);
     h->c = NULL;  /* later calls on this handle throw instead of touching freed memory */
     if (rc) return throw_rc(env, rc, NULL, "aero_destroy");
@@ -547,6 +566,212 @@ static napi_value js_streamlines(napi_env env, napi_callback_info info) {
     return arr;
 }
 
+/* ---- batched tunnels (aero_batch_*; the aero-map sweep of src/ui/aeromap.ts:73-114) ---- */
+
+static void release_batch(batch_t* bt) {
+    if (--bt->refs) return;
+    for (uint32_t i = 0; i < bt->n; i++) free(bt->inst[i]);
+    free(bt->inst);
+    free(bt);
+}
+static void finalize_batch(napi_env env, void* data, void* hint) {
+    (void)env;
+    (void)hint;
+    release_batch((batch_t*)data);
+}
+static void finalize_instance(napi_env env, void* data, void* hint) {
+    (void)env;
+    (void)data;
+    release_batch((batch_t*)hint);
+}
+
+static batch_t* get_batch(napi_env env, napi_value v) {
+    void* p = NULL;
+    napi_valuetype t;
+    if (napi_typeof(env, v, &t) != napi_ok || t != napi_external || napi_get_value_external(env, v, &p) != napi_ok ||
+        !p || !((batch_t*)p)->b) {
+        napi_throw_type_error(env, "AERO_ARGS", "expected a live batch");
+        return NULL;
+    }
+    return (batch_t*)p;
+}
+
+static napi_value throw_batch(napi_env env, int rc, aero_batch* b, const char* what) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "%s failed (%d): %s", what, rc, b ? aero_batch_last_error(b) : aero_global_error());
+    napi_throw_error(env, "AERO_E", buf);
+    return NULL;
+}
+
+/* batchCreate(grids: Float64Array[8k], sims: Float64Array[3k], storage) -> batch */
+static napi_value js_batch_create(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    if (!get_args(env, info, 3, a)) return NULL;
+    size_t ng = 0, ns = 0;
+    const double* g = (const double*)typed(env, a[0], napi_float64_array, &ng);
+    if (!g) return NULL;
+    const double* s = (const double*)typed(env, a[1], napi_float64_array, &ns);
+    if (!s) return NULL;
+    if (ng == 0 || ng % 8 || ns != ng / 8 * 3 || ng / 8 > 4096) {
+        napi_throw_range_error(env, "AERO_ARGS", "grids need 8 values and sims 3 per tunnel (1..4096 tunnels)");
+        return NULL;
+    }
+    int32_t storage = 0;
+    NAPI_CALL(env, napi_get_value_int32(env, a[2], &storage));
+    const uint32_t n = (uint32_t)(ng / 8);
+    aero_grid* grids = (aero_grid*)calloc(n, sizeof(aero_grid));
+    aero_sim* sims = (aero_sim*)calloc(n, sizeof(aero_sim));
+    batch_t* bt = (batch_t*)calloc(1, sizeof(batch_t));
+    handle_t** inst = (handle_t**)calloc(n, sizeof(handle_t*));
+    if (!grids || !sims || !bt || !inst) {
+        free(grids); free(sims); free(bt); free(inst);
+        napi_throw_error(env, "AERO_NAPI", "allocation failed");
+        return NULL;
+    }
+    for (uint32_t i = 0; i < n; i++) {
+        const double* q = g + 8 * i;
+        grids[i] = (aero_grid){(uint32_t)q[0], (uint32_t)q[1], (uint32_t)q[2], (uint32_t)q[3],
+                               (float)q[4],    (float)q[5],    (float)q[6],    (float)q[7]};
+        sims[i] = (aero_sim){(float)s[3 * i], (float)s[3 * i + 1], (float)s[3 * i + 2], 0.0f};
+    }
+    aero_batch* b = NULL;
+    const int rc = aero_batch_create(n, grids, sims, storage, &b);
+    free(sims);
+    if (rc) {
+        free(grids); free(bt); free(inst);
+        return throw_batch(env, rc, NULL, "aero_batch_create");
+    }
+    bt->b = b;
+    bt->n = n;
+    bt->inst = inst;
+    bt->refs = 1;
+    bt->wrapped = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        handle_t* h = (handle_t*)malloc(sizeof(handle_t));
+        if (!h) {
+            for (uint32_t j = 0; j < i; j++) free(inst[j]);
+            aero_batch_destroy(b);
+            free(grids); free(bt); free(inst);
+            napi_throw_error(env, "AERO_NAPI", "allocation failed");
+            return NULL;
+        }
+        h->c = aero_batch_instance(b, i);
+        h->n = grids[i].n;
+        h->nparts = 0;
+        h->owned = 0;
+        inst[i] = h;
+    }
+    free(grids);
+    napi_value ext;
+    if (napi_create_external(env, bt, finalize_batch, NULL, &ext) != napi_ok) {
+        for (uint32_t i = 0; i < n; i++) free(inst[i]);
+        aero_batch_destroy(b);
+        free(bt); free(inst);
+        napi_throw_error(env, "AERO_NAPI", "napi_create_external failed");
+        return NULL;
+    }
+    return ext;
+}
+
+/* batchInstances(batch) -> [context handle per tunnel]; call once per batch */
+static napi_value js_batch_instances(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    if (!get_args(env, info, 1, a)) return NULL;
+    batch_t* bt = get_batch(env, a[0]);
+    if (!bt) return NULL;
+    if (bt->wrapped) {
+        napi_throw_error(env, "AERO_NAPI", "batchInstances may be called once per batch");
+        return NULL;
+    }
+    bt->wrapped = 1;
+    napi_value arr;
+    NAPI_CALL(env, napi_create_array_with_length(env, bt->n, &arr));
+    for (uint32_t i = 0; i < bt->n; i++) {
+        napi_value ext;
+        NAPI_CALL(env, napi_create_external(env, bt->inst[i], finalize_instance, bt, &ext));
+        bt->refs++;
+        NAPI_CALL(env, napi_set_element(env, arr, i, ext));
+    }
+    return arr;
+}
+
+static napi_value js_batch_destroy(napi_env env, napi_callback_info info) {
+    napi_value a[1];
+    if (!get_args(env, info, 1, a)) return NULL;
+    batch_t* bt = get_batch(env, a[0]);
+    if (!bt) return NULL;
+    for (uint32_t i = 0; i < bt->n; i++)
+        if (bt->inst[i]) bt->inst[i]->c = NULL;  /* instance handles throw from now on */
+    const int rc = aero_batch_destroy(bt->b);
+    bt->b = NULL;
+    if (rc) return throw_batch(env, rc, NULL, "aero_batch_destroy");
+    return NULL;
+}
+
+/* batchStep(batch, nsteps) */
+static napi_value js_batch_step(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!get_args(env, info, 2, a)) return NULL;
+    batch_t* bt = get_batch(env, a[0]);
+    if (!bt) return NULL;
+    uint32_t n = 0;
+    NAPI_CALL(env, napi_get_value_uint32(env, a[1], &n));
+    const int rc = aero_batch_step(bt->b, n);
+    if (rc) return throw_batch(env, rc, bt->b, "aero_batch_step");
+    return NULL;
+}
+
+/* batchRunGraph(batch, nsteps, stepsPerGraph) -> device ms */
+static napi_value js_batch_run_graph(napi_env env, napi_callback_info info) {
+    napi_value a[3];
+    if (!get_args(env, info, 3, a)) return NULL;
+    batch_t* bt = get_batch(env, a[0]);
+    if (!bt) return NULL;
+    uint32_t n = 0, spg = 0;
+    NAPI_CALL(env, napi_get_value_uint32(env, a[1], &n));
+    NAPI_CALL(env, napi_get_value_uint32(env, a[2], &spg));
+    float ms = 0.0f;
+    const int rc = aero_batch_run_graph(bt->b, n, spg, &ms);
+    if (rc) return throw_batch(env, rc, bt->b, "aero_batch_run_graph");
+    napi_value v;
+    NAPI_CALL(env, napi_create_double(env, ms, &v));
+    return v;
+}
+
+/* batchAutotune(batch, stepsPerShape) -> { shape, usPerStep } */
+static napi_value js_batch_autotune(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!get_args(env, info, 2, a)) return NULL;
+    batch_t* bt = get_batch(env, a[0]);
+    if (!bt) return NULL;
+    uint32_t k = 0;
+    NAPI_CALL(env, napi_get_value_uint32(env, a[1], &k));
+    int shape = 0;
+    float us = 0.0f;
+    const int rc = aero_batch_autotune(bt->b, k, &shape, &us);
+    if (rc) return throw_batch(env, rc, bt->b, "aero_batch_autotune");
+    napi_value obj, vs, vu;
+    NAPI_CALL(env, napi_create_object(env, &obj));
+    NAPI_CALL(env, napi_create_int32(env, shape, &vs));
+    NAPI_CALL(env, napi_create_double(env, us, &vu));
+    NAPI_CALL(env, napi_set_named_property(env, obj, "shape", vs));
+    NAPI_CALL(env, napi_set_named_property(env, obj, "usPerStep", vu));
+    return obj;
+}
+
+/* batchSetLaunchShape(batch, shape) */
+static napi_value js_batch_set_launch_shape(napi_env env, napi_callback_info info) {
+    napi_value a[2];
+    if (!get_args(env, info, 2, a)) return NULL;
+    batch_t* bt = get_batch(env, a[0]);
+    if (!bt) return NULL;
+    int32_t sh = 0;
+    NAPI_CALL(env, napi_get_value_int32(env, a[1], &sh));
+    const int rc = aero_batch_set_launch_shape(bt->b, sh);
+    if (rc) return throw_batch(env, rc, bt->b, "aero_batch_set_launch_shape");
+    return NULL;
+}
+
 static napi_value js_version(napi_env env, napi_callback_info info) {
     (void)info;
     napi_value v;
@@ -570,6 +795,10 @@ static napi_value init_module(napi_env env, napi_value exports) {
         {"sampleFlow", js_sample_flow}, {"particlesAlloc", js_particles_alloc},
         {"particlesStep", js_particles_step}, {"readParticles", js_read_particles},
         {"writeParticles", js_write_particles}, {"streamlines", js_streamlines},
+        {"batchCreate", js_batch_create}, {"batchInstances", js_batch_instances},
+        {"batchDestroy", js_batch_destroy}, {"batchStep", js_batch_step},
+        {"batchRunGraph", js_batch_run_graph}, {"batchAutotune", js_batch_autotune},
+        {"batchSetLaunchShape", js_batch_set_launch_shape},
     };
     for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
         napi_value f;

@@ -79,3 +79,28 @@ export declare class HipAeroEngine {
   sampleFlow(points: Float32Array, view: Float32Array, colorMode?: number): Float32Array;
   writeFields(f: Float32Array): void;
 }
+
+/** Batched tunnels (aero_batch_*): one engine per instance, stepped together. */
+export interface TunnelSpec { dims: [number, number, number]; h: number; origin?: [number, number, number];
+                              uLb?: number; tau0?: number; smag?: number }
+export declare class HipAeroBatch {
+  constructor(specs: TunnelSpec[], opts?: { storage?: number; kmh?: number });
+  readonly engines: HipAeroEngine[];
+  step(nsteps: number): void;
+  /** device milliseconds of the run */
+  runGraph(nsteps: number, stepsPerGraph?: number): number;
+  autotune(stepsPerShape?: number): { shape: number; usPerStep: number };
+  setLaunchShape(shape: number): void;
+  destroy(): void;
+}
+
+/** The ride-height sweep of src/ui/aeromap.ts (front x rear, metres) as one batch. */
+export declare const AEROMAP_FRONT: number[];
+export declare const AEROMAP_REAR: number[];
+export interface AeroMapCell { front: number; rear: number; down: number; eff: number; bal: number; forces: Forces }
+export declare function aeroMap(opts: {
+  grid: GridSpec; shape: (front: number, rear: number) => ShapeParams;
+  prims?: (front: number, rear: number) => Uint8Array | null; settle: number; measure: number;
+  front?: number[]; rear?: number[]; storage?: number; stepsPerGraph?: number; launchShape?: number;
+  uLb?: number; tau0?: number; smag?: number; kmh?: number;
+}): { cells: AeroMapCell[]; deviceMs: number };

@@ -138,7 +138,7 @@ class HipAeroEngine {
    * @param {string|{dims:number[],h:number,origin:number[]}} quality preset name, or an explicit grid
    * @param {{uLb?:number,tau0?:number,smag?:number,storage?:number,kmh?:number}} [opts]
    */
-  constructor(quality, opts) {
+  constructor(quality, opts, attachCtx) {
     opts = opts || {};
     let dims, h, origin;
     if (typeof quality === "string" || quality === undefined) {
@@ -170,16 +170,17 @@ class HipAeroEngine {
     this.onForces = undefined;
     this.profileMeta = undefined;
     this.profile = null;
-    this.ctx = addon.create(new Float64Array([dims[0], dims[1], dims[2], this.n, origin[0], origin[1], origin[2], h]),
-                            new Float64Array([this.uLb, this.tau0, this.smag]),
-                            opts.storage !== undefined ? opts.storage : Storage.F32);
+    // attachCtx: a context owned by a batch (HipAeroBatch), released with it
+    this.owned = !attachCtx;
+    this.ctx = attachCtx ||
+      addon.create(new Float64Array([dims[0], dims[1], dims[2], this.n, origin[0], origin[1], origin[2], h]),
+                   new Float64Array([this.uLb, this.tau0, this.smag]),
+                   opts.storage !== undefined ? opts.storage : Storage.F32);
   }
 
   destroy() {
-    if (this.ctx) {
-      addon.destroy(this.ctx);
-      this.ctx = null;
-    }
+    if (this.ctx && this.owned) addon.destroy(this.ctx);
+    this.ctx = null;
   }
 
   /** engine.ts:371-381 */
@@ -345,5 +346,88 @@ class HipAeroEngine {
   sync() { addon.sync(this.ctx); }
 }
 
-module.exports = { HipAeroEngine, ShapeKind, Storage, FlowMode, ColorField, packShape, packPSim, packStream, viewUniform,
+/** Batched tunnels (aero_batch_*): one HipAeroEngine per instance (setShape, reset,
+ *  readForces, debugRead work on each), all stepped together by one fused launch per
+ *  step, replayed from a captured HIP graph.  specs: [{dims, h, origin?, uLb?, tau0?,
+ *  smag?}]; one storage mode for all. */
+class HipAeroBatch {
+  constructor(specs, opts) {
+    opts = opts || {};
+    if (!specs || !specs.length) throw new Error("HipAeroBatch needs at least one tunnel");
+    const grids = new Float64Array(8 * specs.length);
+    const sims = new Float64Array(3 * specs.length);
+    specs.forEach((sp, i) => {
+      const d = sp.dims, o = sp.origin || [ORIGIN_X, 0, 0];
+      grids.set([d[0], d[1], d[2], d[0] * d[1] * d[2], o[0], o[1], o[2], sp.h], 8 * i);
+      sims.set([sp.uLb !== undefined ? sp.uLb : LATTICE_U, sp.tau0 !== undefined ? sp.tau0 : TAU0,
+                sp.smag !== undefined ? sp.smag : SMAG], 3 * i);
+    });
+    this.b = addon.batchCreate(grids, sims, opts.storage !== undefined ? opts.storage : Storage.F16S);
+    const ctxs = addon.batchInstances(this.b);
+    this.engines = specs.map((sp, i) => new HipAeroEngine(
+      { dims: sp.dims, h: sp.h, origin: sp.origin }, Object.assign({ kmh: opts.kmh }, sp), ctxs[i]));
+  }
+  /** nsteps fused steps launched one by one */
+  step(nsteps) {
+    addon.batchStep(this.b, nsteps);
+    for (const e of this.engines) e.stepsSinceRead += nsteps;
+  }
+  /** nsteps steps replayed from a graph of stepsPerGraph (even) steps; device ms */
+  runGraph(nsteps, stepsPerGraph) {
+    const ms = addon.batchRunGraph(this.b, nsteps, stepsPerGraph || 100);
+    for (const e of this.engines) e.stepsSinceRead += nsteps;
+    return ms;
+  }
+  autotune(stepsPerShape) { return addon.batchAutotune(this.b, stepsPerShape || 4); }
+  setLaunchShape(shape) { addon.batchSetLaunchShape(this.b, shape); }
+  destroy() {
+    if (this.b) {
+      addon.batchDestroy(this.b);
+      this.b = null;
+      for (const e of this.engines) e.ctx = null;
+    }
+  }
+}
+
+// src/ui/aeromap.ts:15-19
+const AEROMAP_FRONT = [0.08, 0.11, 0.14, 0.17, 0.2];
+const AEROMAP_REAR = [0.1, 0.14, 0.18, 0.22, 0.26];
+const KGF = 9.81;
+
+/** The ride-height map of src/ui/aeromap.ts:73-114 as one batched request: every
+ *  (front, rear) pair is its own tunnel, all settled and measured together instead of
+ *  one after another on a timer.  opts: grid {dims, h, origin}, shape(front, rear) ->
+ *  ShapeParams, prims(front, rear) -> Uint8Array|null, settle / measure steps, storage,
+ *  stepsPerGraph.  Returns the reference's cells: down (kg), eff (L/D), bal (the front
+ *  share of the wing load, which the reference takes from its panel-method wing loads
+ *  on the W17 only; 0.5 otherwise, as there), plus the raw Forces. */
+function aeroMap(opts) {
+  const front = opts.front || AEROMAP_FRONT, rear = opts.rear || AEROMAP_REAR;
+  const pairs = [];
+  front.forEach((f) => rear.forEach((r) => pairs.push([f, r])));
+  const g = opts.grid;
+  const spec = { dims: g.dims, h: g.h, origin: g.origin, uLb: opts.uLb, tau0: opts.tau0, smag: opts.smag };
+  const batch = new HipAeroBatch(pairs.map(() => spec), { storage: opts.storage, kmh: opts.kmh });
+  try {
+    pairs.forEach(([f, r], i) => {
+      const e = batch.engines[i];
+      e.setShape(opts.shape(f, r), opts.prims ? opts.prims(f, r) : null);
+      e.reset();
+    });
+    if (opts.launchShape !== undefined) batch.setLaunchShape(opts.launchShape);
+    let ms = batch.runGraph(opts.settle, opts.stepsPerGraph);
+    for (const e of batch.engines) e.readForces(true);
+    ms += batch.runGraph(opts.measure, opts.stepsPerGraph);
+    const cells = pairs.map(([f, r], i) => {
+      const F = batch.engines[i].readForces(true);
+      return { front: f, rear: r, down: F.downforceN / KGF, eff: F.downforceN / Math.max(F.dragN, 1), bal: 0.5,
+               forces: F };
+    });
+    return { cells, deviceMs: ms };
+  } finally {
+    batch.destroy();
+  }
+}
+
+module.exports = { HipAeroEngine, HipAeroBatch, aeroMap, AEROMAP_FRONT, AEROMAP_REAR, ShapeKind, Storage, FlowMode, ColorField, packShape, packPSim, packStream, viewUniform,
                    addon, QUALITIES, PARTICLES, MAX_LINES, STREAM_STEPS, LATTICE_U, TAU0, SMAG };

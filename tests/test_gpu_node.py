@@ -57,3 +57,34 @@ def test_node_engine_checks_and_python_agreement():
     assert words(parts) == fl["particles"]
     assert words(lines) == fl["lines"]
     assert words(vort) == fl["vortices"]
+
+
+def test_node_batched_aeromap_matches_python_host():
+    """aeroMap (hip-engine.js) against the Python host driving the same batch: the 25
+    cells of the reference's sweep (src/ui/aeromap.ts:15-19,97-101) agree to 1e-12."""
+    print("node test_aeromap.js", flush=True)
+    r = subprocess.run(["node", "test_aeromap.js"], cwd=NODE_DIR, capture_output=True, text=True, timeout=180)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0
+    js = json.loads(r.stdout.strip().splitlines()[-1])
+    from aerolbm import aeromap
+    from aerolbm.batch import AeroBatch
+    spec = dict(dims=(96, 32, 32), h=0.12, origin=(-2.4, 0.0, 0.0))
+    pairs = [(f, r_) for f in aeromap.REFERENCE_FRONT for r_ in aeromap.REFERENCE_REAR]
+    b = AeroBatch([spec] * len(pairs), storage="f16s")
+    for e, (f, r_) in zip(b.engines, pairs):
+        e.set_shape(aerolbm.ShapeParams(kind=aerolbm.ShapeKind.F1, rideH=f, rearRideH=r_, aoaDeg=6, cooling=1,
+                                        heatBrakes=False, heatEngine=False, heatRad=False))
+        e.reset()
+    b.run_graph(40, 20)
+    for e in b.engines:
+        e.read_forces(reset=True)
+    b.run_graph(40, 20)
+    cells = aeromap.map_cells([e.read_forces(reset=True) for e in b.engines], pairs)
+    b.close()
+    assert len(js["cells"]) == len(cells) == 25
+    for c, j in zip(cells, js["cells"]):
+        assert (c["front"], c["rear"]) == (pytest.approx(j["front"]), pytest.approx(j["rear"]))
+        for k in ("down", "eff", "bal"):
+            assert c[k] == pytest.approx(j[k], rel=1e-12, abs=1e-12), k
+    assert len({round(c["down"], 6) for c in cells}) > 5  # ride height changes the load

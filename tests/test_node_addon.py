@@ -35,7 +35,9 @@ try { addon.step({}, 1); } catch (e) { typeMsg = e.message; }
 console.log(JSON.stringify({ keys: Object.keys(addon), version: addon.version(), msg, typeMsg }));
 """))
     for k in ("create", "destroy", "init", "voxelize", "setFlags", "step", "heatStep", "readForces", "readFields",
-              "readTemps", "readFlags", "readPopulations", "guardCount", "sync"):
+              "readTemps", "readFlags", "readPopulations", "guardCount", "sync", "writeFields", "sampleFlow",
+              "particlesAlloc", "particlesStep", "readParticles", "writeParticles", "streamlines", "batchCreate",
+              "batchInstances", "batchDestroy", "batchStep", "batchRunGraph", "batchAutotune", "batchSetLaunchShape"):
         assert k in out["keys"]
     assert out["version"] == aerolbm.load().aero_version()
     assert "aero_create failed" in out["msg"] and "tau0" in out["msg"]
