@@ -160,4 +160,6 @@ hipError_t launch_voxelize(aero_ctx* c, const aero_shape* s, const float* d_prof
 hipError_t launch_pack_pops(aero_ctx* c, const float* d_in);    // f32 SoA -> storage
 hipError_t launch_unpack_pops(aero_ctx* c, float* d_out);       // storage -> f32 SoA
 void fill_step_consts(const aero_ctx* c, StepArgs* a);
+// Message of a failure that has no context (aero_global_error).
+void report_global_error(const std::string& m);
 }  // namespace aero

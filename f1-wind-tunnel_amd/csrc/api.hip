@@ -89,6 +89,8 @@ size_t store_bytes(int storage) { return storage == AERO_STORE_F32 ? 4 : 2; }
 
 }  // namespace
 
+void aero::report_global_error(const std::string& m) { set_global_error(m); }
+
 extern "C" {
 
 uint32_t aero_version(void) { return 1u; }

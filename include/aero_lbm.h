@@ -247,6 +247,19 @@ int aero_particles_write(aero_ctx* ctx, const float* in_count8);
 int aero_streamlines(aero_ctx* ctx, const aero_stream* stream, const aero_view* view, const float* emitters_32,
                      float* out_lines_steps_4);
 
+/* ---- Distance-grid baker ----
+ * bakeGrid of scripts/bake-car.ts:237-316: the f16 distance grids the W17 voxelizer
+ * samples (w17.sdf.bin, w17.parts.bin), computed on the current HIP device from the
+ * car's triangles.  tris holds ntri x 9 FP64 corners in the car frame (the caller
+ * selects the triangles of the grid, bake-car.ts:244-253).  out receives dims[0] x
+ * dims[1] x dims[2] f16 words, x fastest: the distance to the nearest triangle (within
+ * band) minus shell outside, -max(distance, 0.001) - shell inside, where inside is
+ * what a flood from every face but z = 0 cannot reach through cells closer than
+ * `closing` to the surface, grown back by ceil(closing / h) cells. */
+typedef struct { double origin[3]; double h; uint32_t dims[3]; uint32_t pad; } aero_sdf_grid;
+int aero_bake_sdf(const double* tris_9, uint32_t ntri, const aero_sdf_grid* grid, double band, double closing,
+                  double shell, uint16_t* out_f16);
+
 /* ---- Batched tunnels ----
  * Replaces the reference's sequential ride-height sweep (src/ui/aeromap.ts:73-114,
  * one setHeights → settle → measure period per map point on the one engine) with

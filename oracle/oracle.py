@@ -240,3 +240,47 @@ class FlowOracle:
         out = np.empty((stream.lines, stream.steps, 4), np.float32)
         self.lib.fo_streamlines(C.byref(g), _p(f), _p(t), self._s(stream, 64), self._s(view, 32), _p(em), _p(out))
         return out
+
+
+class BakeOracle:
+    """FP64 restatement of the W17 distance-grid baker (bake_oracle.c)."""
+
+    def __init__(self):
+        path = LIBDIR / "libbake_oracle.so"
+        if not path.exists():
+            build()
+        L = self.lib = C.CDLL(str(path))
+        vp = C.c_void_p
+        L.bo_closest.restype = C.c_double
+        L.bo_closest.argtypes = [C.c_double, C.c_double, C.c_double, vp]
+        L.bo_to_half.restype = C.c_uint16
+        L.bo_to_half.argtypes = [C.c_double]
+        L.bo_distance.argtypes = [vp, C.c_uint32, vp, C.c_double, vp, C.c_double, vp]
+        L.bo_bake_grid.restype = C.c_int
+        L.bo_bake_grid.argtypes = [vp, C.c_uint32, vp, C.c_double, vp, C.c_double, C.c_double, C.c_double, vp]
+
+    def closest(self, p, tri) -> float:
+        t = np.ascontiguousarray(tri, dtype=np.float64).reshape(9)
+        return float(self.lib.bo_closest(float(p[0]), float(p[1]), float(p[2]), _p(t)))
+
+    def to_half(self, v: float) -> int:
+        return int(self.lib.bo_to_half(float(v)))
+
+    def distance(self, tris, layout, band) -> np.ndarray:
+        t = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+        o = np.asarray(layout["origin"], np.float64)
+        d = np.asarray(layout["dims"], np.uint32)
+        out = np.empty(int(np.prod(d)), np.float32)
+        self.lib.bo_distance(_p(t), len(t), _p(o), float(layout["h"]), _p(d), float(band), _p(out))
+        return out
+
+    def bake(self, tris, layout, band, closing, shell) -> np.ndarray:
+        t = np.ascontiguousarray(tris, dtype=np.float64).reshape(-1, 9)
+        o = np.asarray(layout["origin"], np.float64)
+        d = np.asarray(layout["dims"], np.uint32)
+        out = np.empty(int(np.prod(d)), np.uint16)
+        rc = self.lib.bo_bake_grid(_p(t), len(t), _p(o), float(layout["h"]), _p(d), float(band), float(closing),
+                                   float(shell), _p(out))
+        if rc != 0:
+            raise MemoryError("bo_bake_grid")
+        return out

@@ -125,16 +125,53 @@ def w17_rig_fixture() -> None:
     (HERE / "w17_rig.json").write_text(json.dumps(keep, separators=(",", ":")) + "\n")
 
 
+def w17_part_grids_fixture() -> dict:
+    """Two of the reference's baked part grids (public/models/w17.parts.bin, a data
+    file written by the reference's own baker, scripts/bake-car.ts) with their inputs:
+    the FP64 car-frame triangles of the front flap A (part 1) and the rear flap (part 3)
+    selected from public/models/w17.glb with the part tags of w17.rig.bin, the grid
+    layouts of w17.rig.json, and the baker's band / closing / shell.  The expected words
+    are the reference's, not the oracle's.  Written only where the reference is present;
+    the committed copy serves everywhere else."""
+    import json
+
+    from aerolbm import bake as B
+
+    if not (REF_MODELS / "w17.glb").exists():
+        return None
+    tags = np.fromfile(REF_MODELS / "w17.rig.bin", dtype=np.uint8)
+    prims = B.load_prims(REF_MODELS / "w17.glb")
+    parts_bin = np.fromfile(REF_MODELS / "w17.parts.bin", dtype="<u2")
+    grids = {g["part"]: g for g in json.loads((REF_MODELS / "w17.rig.json").read_text())["partGrids"]}
+    out = {}
+    for part, want, closing in B.PART_SPECS:
+        if part not in (1, 3):
+            continue
+        g = grids[part]
+        n = int(np.prod(g["dims"]))
+        out[f"p{part}_tris"] = B.select_triangles(prims, tags, want)
+        out[f"p{part}_origin"] = np.array(g["origin"], np.float64)
+        out[f"p{part}_h"] = np.float64(g["h"])
+        out[f"p{part}_dims"] = np.array(g["dims"], np.uint32)
+        out[f"p{part}_params"] = np.array([max(0.03, closing + 0.02), closing, B.PART_SHELL], np.float64)
+        out[f"p{part}_expected"] = parts_bin[g["offset"]:g["offset"] + n].copy()
+    return out
+
+
 def main() -> None:
     lbm = lbm_fixture()
     files = {"lbm_mixed_24x12x8.npz": lbm, "heat_24x12x8.npz": heat_fixture(lbm), "voxelize_small.npz": vox_fixture()}
     w17_rig_fixture()
+    parts = w17_part_grids_fixture()
+    if parts is not None:
+        files["w17_part_grids.npz"] = parts
     lines = []
     for name, data in files.items():
         np.savez_compressed(HERE / name, **data)
         lines.append(f"{hashlib.sha256((HERE / name).read_bytes()).hexdigest()}  {name}")
-    name = "w17_rig.json"
-    lines.append(f"{hashlib.sha256((HERE / name).read_bytes()).hexdigest()}  {name}")
+    for name in ("w17_rig.json", "w17_part_grids.npz"):
+        if not any(ln.endswith("  " + name) for ln in lines):
+            lines.append(f"{hashlib.sha256((HERE / name).read_bytes()).hexdigest()}  {name}")
     (HERE / "MANIFEST.sha256").write_text("\n".join(lines) + "\n")
     print("\n".join(lines))
 
