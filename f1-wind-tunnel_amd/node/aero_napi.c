@@ -772,6 +772,45 @@ static napi_value js_batch_set_launch_shape(napi_env env, napi_callback_info inf
     return NULL;
 }
 
+/* bakeSdf(tris: Float64Array[9k], origin: Float64Array[3], h, dims: Uint32Array[3], band, closing,
+ *         shell) -> Uint16Array (the f16 words of bakeGrid, scripts/bake-car.ts:237-316) */
+static napi_value js_bake_sdf(napi_env env, napi_callback_info info) {
+    napi_value a[7];
+    if (!get_args(env, info, 7, a)) return NULL;
+    size_t nt = 0, no = 0, nd = 0;
+    const double* tris = (const double*)typed(env, a[0], napi_float64_array, &nt);
+    if (!tris) return NULL;
+    const double* org = (const double*)typed(env, a[1], napi_float64_array, &no);
+    if (!org) return NULL;
+    const uint32_t* dims = (const uint32_t*)typed(env, a[3], napi_uint32_array, &nd);
+    if (!dims) return NULL;
+    if (nt % 9 || nt / 9 > 0xffffffffu || no != 3 || nd != 3) {
+        napi_throw_range_error(env, "AERO_ARGS", "tris must be whole 9-value triangles, origin and dims 3 values");
+        return NULL;
+    }
+    double h = 0, band = 0, closing = 0, shell = 0;
+    NAPI_CALL(env, napi_get_value_double(env, a[2], &h));
+    NAPI_CALL(env, napi_get_value_double(env, a[4], &band));
+    NAPI_CALL(env, napi_get_value_double(env, a[5], &closing));
+    NAPI_CALL(env, napi_get_value_double(env, a[6], &shell));
+    const uint64_t n = (uint64_t)dims[0] * dims[1] * dims[2];
+    if (n == 0 || n > (1ull << 31)) {
+        napi_throw_range_error(env, "AERO_ARGS", "grid must hold 1 .. 2^31 cells");
+        return NULL;
+    }
+    aero_sdf_grid g;
+    memset(&g, 0, sizeof g);
+    memcpy(g.origin, org, sizeof g.origin);
+    g.h = h;
+    memcpy(g.dims, dims, sizeof g.dims);
+    void* p = NULL;
+    napi_value arr = make_typed(env, napi_uint16_array, (size_t)n, 2, &p);
+    if (!arr) return NULL;
+    const int rc = aero_bake_sdf(nt ? tris : NULL, (uint32_t)(nt / 9), &g, band, closing, shell, (uint16_t*)p);
+    if (rc) return throw_rc(env, rc, NULL, "aero_bake_sdf");
+    return arr;
+}
+
 static napi_value js_version(napi_env env, napi_callback_info info) {
     (void)info;
     napi_value v;
@@ -798,7 +837,7 @@ static napi_value init_module(napi_env env, napi_value exports) {
         {"batchCreate", js_batch_create}, {"batchInstances", js_batch_instances},
         {"batchDestroy", js_batch_destroy}, {"batchStep", js_batch_step},
         {"batchRunGraph", js_batch_run_graph}, {"batchAutotune", js_batch_autotune},
-        {"batchSetLaunchShape", js_batch_set_launch_shape},
+        {"batchSetLaunchShape", js_batch_set_launch_shape}, {"bakeSdf", js_bake_sdf},
     };
     for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
         napi_value f;

@@ -104,3 +104,8 @@ export declare function aeroMap(opts: {
   front?: number[]; rear?: number[]; storage?: number; stepsPerGraph?: number; launchShape?: number;
   uLb?: number; tau0?: number; smag?: number; kmh?: number;
 }): { cells: AeroMapCell[]; deviceMs: number };
+
+/** bakeGrid of scripts/bake-car.ts:237-316 on the GPU (aero_bake_sdf): f16 words, x fastest. */
+export declare function bakeGrid(tris: Float64Array, layout: { origin: [number, number, number]; h: number;
+                                 dims: [number, number, number] }, band: number, closing: number,
+                                 shell: number): Uint16Array;

@@ -429,5 +429,13 @@ function aeroMap(opts) {
   }
 }
 
-module.exports = { HipAeroEngine, HipAeroBatch, aeroMap, AEROMAP_FRONT, AEROMAP_REAR, ShapeKind, Storage, FlowMode, ColorField, packShape, packPSim, packStream, viewUniform,
+/** bakeGrid of scripts/bake-car.ts:237-316 on the GPU: the f16 distance grid (x fastest)
+ *  of the given triangles (Float64Array, 9 car-frame coordinates each, already selected
+ *  by part), for layout {origin, h, dims}. */
+function bakeGrid(tris, layout, band, closing, shell) {
+  return addon.bakeSdf(tris, new Float64Array(layout.origin), layout.h, new Uint32Array(layout.dims), band, closing,
+                       shell);
+}
+
+module.exports = { HipAeroEngine, HipAeroBatch, aeroMap, bakeGrid, AEROMAP_FRONT, AEROMAP_REAR, ShapeKind, Storage, FlowMode, ColorField, packShape, packPSim, packStream, viewUniform,
                    addon, QUALITIES, PARTICLES, MAX_LINES, STREAM_STEPS, LATTICE_U, TAU0, SMAG };

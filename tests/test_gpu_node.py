@@ -88,3 +88,33 @@ def test_node_batched_aeromap_matches_python_host():
         for k in ("down", "eff", "bal"):
             assert c[k] == pytest.approx(j[k], rel=1e-12, abs=1e-12), k
     assert len({round(c["down"], 6) for c in cells}) > 5  # ride height changes the load
+
+
+def test_node_bake_grid_reproduces_the_reference_part_grid():
+    """bakeGrid (hip-engine.js over aero_bake_sdf) on the committed triangles of the W17
+    rear flap: the reference's own baked grid, bit for bit."""
+    import tempfile
+
+    gold = ROOT / "tests" / "golden" / "w17_part_grids.npz"
+    with np.load(gold, allow_pickle=False) as d:
+        tris, origin, h = d["p3_tris"], d["p3_origin"], float(d["p3_h"])
+        dims, params, want = d["p3_dims"], d["p3_params"], d["p3_expected"]
+    with tempfile.TemporaryDirectory() as tmp:
+        tp = Path(tmp) / "tris.f64"
+        op = Path(tmp) / "out.u16"
+        np.ascontiguousarray(tris, np.float64).tofile(tp)
+        script = f"""
+const fs = require('fs');
+const {{ bakeGrid }} = require('./hip-engine');
+const b = fs.readFileSync({json.dumps(str(tp))});
+const tris = new Float64Array(b.buffer, b.byteOffset, b.byteLength / 8);
+const out = bakeGrid(tris, {{origin: {json.dumps([float(v) for v in origin])}, h: {h!r},
+                            dims: {json.dumps([int(v) for v in dims])}}}, {float(params[0])!r}, {float(params[1])!r},
+                     {float(params[2])!r});
+fs.writeFileSync({json.dumps(str(op))}, Buffer.from(out.buffer, out.byteOffset, out.byteLength));
+"""
+        r = subprocess.run(["node", "-e", script], cwd=NODE_DIR, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        got = np.fromfile(op, dtype=np.uint16)
+    assert np.array_equal(got, want)
+

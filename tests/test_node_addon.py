@@ -37,7 +37,8 @@ console.log(JSON.stringify({ keys: Object.keys(addon), version: addon.version(),
     for k in ("create", "destroy", "init", "voxelize", "setFlags", "step", "heatStep", "readForces", "readFields",
               "readTemps", "readFlags", "readPopulations", "guardCount", "sync", "writeFields", "sampleFlow",
               "particlesAlloc", "particlesStep", "readParticles", "writeParticles", "streamlines", "batchCreate",
-              "batchInstances", "batchDestroy", "batchStep", "batchRunGraph", "batchAutotune", "batchSetLaunchShape"):
+              "batchInstances", "batchDestroy", "batchStep", "batchRunGraph", "batchAutotune", "batchSetLaunchShape",
+              "bakeSdf"):
         assert k in out["keys"]
     assert out["version"] == aerolbm.load().aero_version()
     assert "aero_create failed" in out["msg"] and "tau0" in out["msg"]
