@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload f1|cube|large]
                     [--storage f32|f16s|bf16s] [--math fast|exact] [--no-cpu]
-                    [--no-aeromap] [--no-surrogate] [--no-flow]
+                    [--no-aeromap] [--no-surrogate] [--no-flow] [--no-wings]
 
 Workload (default "f1"): the generic F1 body, 320x96x96 half tunnel at 4.5 cm
 cells, FP32 populations, D3Q19 regularized + Smagorinsky, moving floor, sponge
@@ -183,6 +183,46 @@ def flow_bench(e, dispatches: int = 200) -> dict:
     return out
 
 
+def wing_loads_bench(calls: int = 20, cpu: bool = True) -> dict:
+    """The W17 wing loads (the reference's panel method, src/engine/vlm.ts) at the 25
+    ride-height pairs of the aero map, 250 km/h: 50 lattices (front and rear wing, 240
+    horseshoes each) per aero_vlm_loads call, timed over `calls` calls (each uploads
+    the lattices, solves and reads back; lattices are built once on the host).  The
+    wing stations are the reference's W17 data (public/models/w17.rig.json, kept as
+    tests/golden/w17_rig.json).  The CPU baseline is the FP64 oracle (one thread) on
+    one pair's two lattices."""
+    from aerolbm import aeromap as AM
+    from aerolbm import vlm as V
+
+    rig = json.loads((ROOT / "tests" / "golden" / "w17_rig.json").read_text())
+    pairs = [(f, r) for f in AM.REFERENCE_FRONT for r in AM.REFERENCE_REAR]
+    lattices, speeds = [], []
+    for f, r in pairs:
+        fl, rl = V.w17_lattices(rig, V.W17WingSetup(rideH=f, rearRideH=r))
+        lattices += [fl, rl]
+        speeds += [250 / 3.6, 250 / 3.6]
+    V.solve_lattices(lattices, speeds)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        loads, _ = V.solve_lattices(lattices, speeds)
+    ms = (time.perf_counter() - t0) / calls * 1e3
+    out = {"lattices": len(lattices), "horseshoes": int(sum(len(h) for h in lattices)),
+           "ms_per_call": round(ms, 3), "lattices_per_s": round(len(lattices) / (ms / 1e3), 1),
+           "front_down_N": round(loads[0].downforceN, 3), "rear_down_N": round(loads[1].downforceN, 3)}
+    if cpu:
+        from oracle import oracle as orc
+
+        vo = orc.VLMOracle()
+        t0 = time.perf_counter()
+        for h, s in zip(lattices[:2], speeds[:2]):
+            vo.loads(h, s)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(2 / dt, 2), "unit": "lattices/s", "cores": 1, "kind": "port",
+                               "sample": f"one ride-height pair (front + rear lattice, 480 horseshoes) through "
+                                         f"the FP64 oracle, {dt * 1e3:.1f} ms"}
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -195,6 +235,7 @@ def main() -> None:
     ap.add_argument("--no-aeromap", action="store_true")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-flow", action="store_true")
+    ap.add_argument("--no-wings", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -301,6 +342,9 @@ def main() -> None:
         result["aeromap"] = aeromap_bench()
     if not args.no_surrogate:
         result["surrogate"] = surrogate_bench()
+    if not args.no_wings:
+        print("[bench] wing loads (vortex lattice, 50 lattices per call)", file=sys.stderr, flush=True)
+        result["wing_loads"] = wing_loads_bench(cpu=not args.no_cpu)
     if not args.no_cpu:
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(wl)

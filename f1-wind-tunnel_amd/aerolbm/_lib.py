@@ -15,6 +15,10 @@ PKG_ROOT = Path(__file__).resolve().parent.parent
 LIB_PATH = Path(os.environ.get("AERO_LBM_LIB", PKG_ROOT / "lib" / "libaero_lbm.so"))
 
 AERO_OK = 0
+AERO_E_INVALID = -1
+AERO_E_NOMEM = -2
+AERO_E_HIP = -3
+AERO_E_STATE = -4
 ERRORS = {-1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "bad state"}
 
 STORE_F32, STORE_F16S, STORE_BF16S = 0, 1, 2
@@ -119,6 +123,7 @@ _PROTOS = {
     "aero_sync": (C.c_int, [_P]),
     "aero_write_fields": (C.c_int, [_P, _P]),
     "aero_bake_sdf": (C.c_int, [_P, _U32, _P, C.c_double, C.c_double, C.c_double, _P]),
+    "aero_vlm_loads": (C.c_int, [_P, _P, _U32, _P, _P, _P]),
     "aero_sample_flow": (C.c_int, [_P, _P, _U32, C.POINTER(View), _U32, _P]),
     "aero_particles_alloc": (C.c_int, [_P, _U32]),
     "aero_particles_step": (C.c_int, [_P, C.POINTER(PSim), C.POINTER(View)]),

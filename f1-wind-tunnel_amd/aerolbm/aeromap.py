@@ -150,6 +150,20 @@ def map_cells(forces, pairs, balance=None) -> List[dict]:
     return out
 
 
+def wing_balance(rig: dict, setup, pairs, speed: float) -> List[Optional[float]]:
+    """The map's `bal` on the W17 (src/ui/aeromap.ts:101 through the balance hook,
+    main.ts:309 and :618-625): the front axle's share of the panel-method wing
+    downforce at each (front, rear) ride-height pair, None where the wings carry less
+    than 100 N.  `setup` is a vlm.W17WingSetup whose ride heights are replaced per
+    pair; every pair's front and rear lattices are solved in one GPU call."""
+    from dataclasses import replace
+
+    from . import vlm as V
+
+    setups = [replace(setup, rideH=float(f), rearRideH=float(r)) for f, r in pairs]
+    return [V.front_axle_share(ld) for ld in V.w17_wing_loads_many(rig, setups, [speed] * len(setups))]
+
+
 def force_errors(run: MapRun, ref: MapRun) -> np.ndarray:
     """Per point: |dF| / |F_ref| with F = (<Fx>, <Fy>), the error measure of the
     map's two coefficients that stays defined when one of them is near zero."""

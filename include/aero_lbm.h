@@ -260,6 +260,21 @@ typedef struct { double origin[3]; double h; uint32_t dims[3]; uint32_t pad; } a
 int aero_bake_sdf(const double* tris_9, uint32_t ntri, const aero_sdf_grid* grid, double band, double closing,
                   double shell, uint16_t* out_f16);
 
+/* ---- Vortex-lattice wing loads ----
+ * wingLoads of src/engine/vlm.ts:140-182 after buildLattice (:86-113), for `count`
+ * lattices (wing setups) at once on the current HIP device.  Lattice l is the
+ * horseshoes offsets[l] .. offsets[l + 1] - 1 of hs (offsets[0] = 0, at most
+ * AERO_VLM_MAX_HORSESHOES each), 20 doubles per horseshoe in the tunnel frame: bound
+ * vortex a, b; trailing-edge points ta, tb; collocation point cp; unit normal n; panel
+ * area; one unused.  The free stream is speeds[l] along +x; the ground plane y = 0 is
+ * modelled by images.  out receives count x 4 doubles (the reference's WingLoads:
+ * downforceN, dragN, copX, dragY); gamma (nullable) the offsets[count] circulations.
+ * FP64 throughout, bit-identical to the reference's evaluation order with V8's
+ * Math.hypot. */
+#define AERO_VLM_MAX_HORSESHOES 1024
+int aero_vlm_loads(const double* hs_20, const uint32_t* offsets, uint32_t count, const double* speeds,
+                   double* out_4, double* gamma);
+
 /* ---- Batched tunnels ----
  * Replaces the reference's sequential ride-height sweep (src/ui/aeromap.ts:73-114,
  * one setHeights → settle → measure period per map point on the one engine) with

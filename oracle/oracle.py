@@ -284,3 +284,56 @@ class BakeOracle:
         if rc != 0:
             raise MemoryError("bo_bake_grid")
         return out
+
+
+class VLMOracle:
+    """FP64 restatement of the vortex-lattice wing loads (vlm_oracle.c).  Horseshoes are
+    (n, 20) float64 rows: a, b, ta, tb, cp, normal, area, pad."""
+
+    def __init__(self):
+        path = LIBDIR / "libvlm_oracle.so"
+        if not path.exists():
+            build()
+        L = self.lib = C.CDLL(str(path))
+        vp = C.c_void_p
+        L.vo_hypot3.restype = C.c_double
+        L.vo_hypot3.argtypes = [C.c_double] * 3
+        L.vo_segment.argtypes = [vp, vp, vp, vp, C.c_double]
+        L.vo_horseshoe.argtypes = [vp, vp, C.c_double, vp]
+        L.vo_solve.argtypes = [vp, vp, C.c_int, vp]
+        L.vo_loads.restype = C.c_int
+        L.vo_loads.argtypes = [vp, C.c_int, C.c_double, vp, vp]
+
+    def hypot3(self, x, y, z) -> float:
+        return self.lib.vo_hypot3(float(x), float(y), float(z))
+
+    def segment(self, p, a, b, gamma=1.0) -> np.ndarray:
+        out = np.zeros(3)
+        arrs = [np.ascontiguousarray(v, np.float64) for v in (p, a, b)]
+        self.lib.vo_segment(_p(arrs[0]), _p(arrs[1]), _p(arrs[2]), _p(out), float(gamma))
+        return out
+
+    def horseshoe(self, p, h, gamma=1.0) -> np.ndarray:
+        out = np.zeros(3)
+        pp = np.ascontiguousarray(p, np.float64)
+        hh = np.ascontiguousarray(h, np.float64)
+        self.lib.vo_horseshoe(_p(pp), _p(hh), float(gamma), _p(out))
+        return out
+
+    def solve(self, A, rhs) -> np.ndarray:
+        a = np.array(A, np.float64, copy=True, order="C")
+        r = np.array(rhs, np.float64, copy=True)
+        x = np.zeros(len(r))
+        self.lib.vo_solve(_p(a), _p(r), len(r), _p(x))
+        return x
+
+    def loads(self, hs, speed):
+        """(downforce N, drag N, copX, dragY), circulations."""
+        h = np.ascontiguousarray(hs, np.float64).reshape(-1, 20)
+        if len(h) == 0:
+            h = np.zeros((1, 20))[:0]
+        out = np.zeros(4)
+        g = np.zeros(max(len(h), 1))
+        if self.lib.vo_loads(_p(h), len(h), float(speed), _p(out), _p(g)) != 0:
+            raise MemoryError("vo_loads")
+        return out, g[:len(h)]

@@ -112,16 +112,17 @@ REF_MODELS = Path("/root/reference/public/models")
 
 def w17_rig_fixture() -> None:
     """The articulation data of the reference's W17 model (public/models/w17.rig.json,
-    a data file): its hinges and part-grid placements, without the render-only wing
-    stations and sections.  Written only where the reference is present; the committed
-    copy serves everywhere else."""
+    a data file): its hinges, part-grid placements and the wing stations the panel
+    method builds its lattices from, without the render-only outline sections.
+    Written only where the reference is present; the committed copy serves everywhere
+    else."""
     src = REF_MODELS / "w17.rig.json"
     if not src.exists():
         return
     import json
 
     rig = json.loads(src.read_text())
-    keep = {k: rig[k] for k in ("frontHinges", "rearMainHinge", "rearFlapHinge", "partGrids")}
+    keep = {k: rig[k] for k in ("frontHinges", "rearMainHinge", "rearFlapHinge", "partGrids", "wings")}
     (HERE / "w17_rig.json").write_text(json.dumps(keep, separators=(",", ":")) + "\n")
 
 
@@ -158,6 +159,42 @@ def w17_part_grids_fixture() -> dict:
     return out
 
 
+def js_hypot_fixture():
+    """Math.hypot(x, y, z) and Math.hypot(x, y) as Node's JavaScript engine (V8)
+    evaluates them, on seeded inputs over many magnitudes plus zeros, subnormals,
+    infinities and NaN.  The reference's panel method (src/engine/vlm.ts) takes every
+    vector length from Math.hypot.  Written only where `node` is installed; the
+    committed copy serves everywhere else."""
+    import json
+    import shutil
+    import subprocess
+    import tempfile
+
+    if shutil.which("node") is None:
+        return None
+    rng = np.random.default_rng(20261203)
+    v = rng.normal(size=(1800, 3)) * 10.0 ** rng.integers(-12, 12, size=(1800, 1))
+    v[:200] *= np.array([1.0, 1e-9, 1e-4])
+    special = [[0, 0, 0], [-0.0, 0, 0], [3, 4, 12], [1e-310, 2e-310, 0], [1e300, 1e300, 1e300],
+               [np.inf, 1, 2], [1, -np.inf, np.nan], [np.nan, 1, 2], [0, 0, 5e-324], [-1, -2, -3]]
+    v = np.concatenate([v, np.array(special, np.float64)])
+    with tempfile.TemporaryDirectory() as d:
+        src = Path(d) / "in.json"
+        dst = Path(d) / "out.bin"
+        def js_num(x):
+            return "NaN" if x != x else ("Infinity" if x == np.inf else ("-Infinity" if x == -np.inf else repr(x)))
+
+        src.write_text(json.dumps([[js_num(float(x)) for x in r] for r in v]))
+        js = ("const fs=require('fs');const v=JSON.parse(fs.readFileSync(process.argv[1]));"
+              "const o=new Float64Array(2*v.length);"
+              "for(let i=0;i<v.length;i++){const a=v[i].map(Number);"
+              "o[2*i]=Math.hypot(a[0],a[1],a[2]);o[2*i+1]=Math.hypot(a[0],a[1]);}"
+              "fs.writeFileSync(process.argv[2],Buffer.from(o.buffer));")
+        subprocess.run(["node", "-e", js, str(src), str(dst)], check=True)
+        o = np.fromfile(dst, np.float64).reshape(-1, 2)
+    return {"inputs": v, "hypot3": o[:, 0].copy(), "hypot2": o[:, 1].copy()}
+
+
 def main() -> None:
     lbm = lbm_fixture()
     files = {"lbm_mixed_24x12x8.npz": lbm, "heat_24x12x8.npz": heat_fixture(lbm), "voxelize_small.npz": vox_fixture()}
@@ -165,11 +202,14 @@ def main() -> None:
     parts = w17_part_grids_fixture()
     if parts is not None:
         files["w17_part_grids.npz"] = parts
+    hyp = js_hypot_fixture()
+    if hyp is not None:
+        files["js_hypot.npz"] = hyp
     lines = []
     for name, data in files.items():
         np.savez_compressed(HERE / name, **data)
         lines.append(f"{hashlib.sha256((HERE / name).read_bytes()).hexdigest()}  {name}")
-    for name in ("w17_rig.json", "w17_part_grids.npz"):
+    for name in ("w17_rig.json", "w17_part_grids.npz", "js_hypot.npz"):
         if not any(ln.endswith("  " + name) for ln in lines):
             lines.append(f"{hashlib.sha256((HERE / name).read_bytes()).hexdigest()}  {name}")
     (HERE / "MANIFEST.sha256").write_text("\n".join(lines) + "\n")

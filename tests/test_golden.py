@@ -21,7 +21,7 @@ def load(name):
 def test_manifest_hashes():
     lines = (GOLD / "MANIFEST.sha256").read_text().split("\n")
     entries = [ln.split() for ln in lines if ln.strip()]
-    assert len(entries) == 5
+    assert len(entries) == 6
     for digest, name in entries:
         assert hashlib.sha256((GOLD / name).read_bytes()).hexdigest() == digest, name
 
