@@ -59,7 +59,15 @@ def build(verbose: bool = False) -> Path:
         list(ex.map(_run, cmds))
     out = LIB / "libaero_lbm.so"
     tmp = LIB / "libaero_lbm.so.tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)])
+    # Export the C ABI only.  Internal C++ symbols (kernel stubs, launchers) stay local, so
+    # the library neither binds to nor lends them to another library that happens to
+    # define the same names: a kernel's host stub address is what the HIP runtime keys a
+    # launch on, and a stub interposed across libraries can launch the other library's
+    # kernel with this one's argument layout.
+    exports = OBJ / "exports.map"
+    exports.write_text("{ global: aero_*; local: *; };\n")
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", f"-Wl,--version-script={exports}", "-o", str(tmp),
+          *map(str, objs)])
     os.replace(tmp, out)
     if verbose:
         print(f"built {out}")

@@ -7,20 +7,23 @@
 //                 velocity A[i][j] of horseshoe j (unit circulation, with its ground
 //                 image) at cp_i, and rhs[i] = -vinf . n_i;
 //   k_vlm_solve   one 1024-thread workgroup per lattice: Gaussian elimination with
-//                 partial pivoting on the L2-resident matrix, the pivot column kept in
-//                 LDS and handed from one step to the next, then back substitution;
+//                 partial pivoting, blocked: panels of up to 16 columns eliminated in
+//                 LDS, the columns right of a panel updated once per panel from one
+//                 L2-resident copy of the matrix into the other, then back substitution
+//                 from LDS-staged rows of U;
 //   k_vlm_forces  one workgroup per bound vortex: the 10 n segment velocities at its
 //                 midpoint computed in parallel into LDS, then added to the free stream
 //                 in the reference's order by three lanes (one per component), and the
 //                 Kutta-Joukowski force and moment terms of that vortex;
-//   k_vlm_sum     one lane per lattice: the force, moment, area and centroid sums in
-//                 index order and the four outputs.
+//   k_vlm_sum     one workgroup per lattice: the force, moment, area and centroid sums
+//                 in index order and the four outputs.
 // Every value is produced by the reference's FP64 operations in the reference's order
 // (no FMA contraction, IEEE division and square root, Math.hypot as V8 evaluates it),
-// so the outputs are bit-identical to a sequential evaluation.  The elimination's
-// updates are independent per matrix entry within one pivot step, so running them in
-// parallel changes no rounding; entries that the reference updates but never reads
-// again (below the diagonal) are left alone.
+// so the outputs are bit-identical to a sequential evaluation.  Within one pivot step
+// the elimination's updates are independent per matrix entry, and the blocked form
+// applies each entry's updates in the same order, so parallel execution changes no
+// rounding; entries that the reference updates but never reads again (below the
+// diagonal) are not kept.
 #include <string>
 #include <vector>
 
@@ -37,14 +40,20 @@ constexpr int VLM_REC = 20;                     // doubles per horseshoe record
 constexpr int VLM_SOLVE_THREADS = 1024;
 constexpr int VLM_SOLVE_WAVES = VLM_SOLVE_THREADS / 64;
 constexpr int VLM_FORCE_THREADS = 256;
-constexpr int VLM_FORCE_CHUNK = 2048;  // segments per LDS pass of k_vlm_forces
+constexpr int VLM_FORCE_CHUNK = 1024;  // segments per LDS pass of k_vlm_forces
+constexpr int VLM_PANEL_W = 16;        // widest elimination panel
+constexpr int VLM_PANEL_CAP = 4096;    // doubles of the LDS panel (rows x width)
+constexpr int VLM_SUM_CHUNK = 512;     // horseshoes per LDS pass of k_vlm_sum
+constexpr int VLM_ROW_BATCH = 8;       // trailing-update rows loaded together per thread
 
 struct VlmBatch {
     const double* hs;       // horseshoe records, all lattices back to back
     const uint32_t* off;    // count + 1 record offsets
     const uint64_t* moff;   // count matrix offsets (doubles)
     const double* speed;    // count free-stream speeds
-    double* A;              // matrices, row-major n x n each
+    double* A;              // matrices, row-major n x n each (k_vlm_matrix's output)
+    double* A2;             // second copy for the blocked elimination
+    double* U;              // the eliminated upper triangles
     double* rhs;            // off[count]
     double* gamma;          // off[count]
     double* terms;          // off[count] x 5: force x, y, z, moment x, moment y terms
@@ -152,115 +161,308 @@ __global__ __launch_bounds__(256) void k_vlm_matrix(VlmBatch B) {
     }
 }
 
-// solve (vlm.ts:115-138) on one lattice per workgroup.
+// Pivot choice across lanes: the largest |value| (NaN never offered), ties to the
+// lowest row; lanes without a candidate hold (-1, INT_MAX).  Within each row of 16
+// lanes by DPP permutes (xor 1, xor 2, mirror of 8, mirror of 16), then across the four
+// rows by lane reads, so every lane ends with the wave's choice.
+template <int CTRL>
+__device__ __forceinline__ void vlm_argmax_step(double& v, int& i) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    const int oi = __builtin_amdgcn_update_dpp(0, i, CTRL, 0xf, 0xf, false);
+    const double ov = __hiloint2double(hi, lo);
+    if (ov > v || (ov == v && oi < i)) {
+        v = ov;
+        i = oi;
+    }
+}
+
+__device__ __forceinline__ void vlm_argmax_row16(double& v, int& i) {
+    vlm_argmax_step<0xB1>(v, i);   // quad_perm [1, 0, 3, 2]
+    vlm_argmax_step<0x4E>(v, i);   // quad_perm [2, 3, 0, 1]
+    vlm_argmax_step<0x141>(v, i);  // row_half_mirror
+    vlm_argmax_step<0x140>(v, i);  // row_mirror
+}
+
+__device__ __forceinline__ void vlm_argmax(double& v, int& i) {
+    vlm_argmax_row16(v, i);
+    double bv = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 0),
+                                 __builtin_amdgcn_readlane(__double2loint(v), 0));
+    int bi = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+    for (int r = 1; r < 4; r++) {
+        const double ov = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 16 * r),
+                                           __builtin_amdgcn_readlane(__double2loint(v), 16 * r));
+        const int oi = __builtin_amdgcn_readlane(i, 16 * r);
+        if (ov > bv || (ov == bv && oi < bi)) {
+            bv = ov;
+            bi = oi;
+        }
+    }
+    v = bv;
+    i = bi;
+}
+
+// solve (vlm.ts:115-138), one lattice per workgroup, as a blocked elimination that
+// keeps the reference's operation sequence on every entry.  The reference eliminates
+// one column per step and updates the whole trailing matrix at once; here a panel of
+// nb columns (rows K .. n-1) is eliminated in LDS, each row's multipliers kept in the
+// panel where the eliminated entries were, and the columns right of the panel receive
+// the panel's nb steps together afterwards: entry (i, j) still gets
+// A[i][j] - f_ik * U[k][j] for k = 0, 1, ... in order, each product and difference
+// rounded on its own, with the same row swaps (a row's pending updates travel with it
+// through its multipliers) and the same skip of zero multipliers.  The trailing block
+// is read through the panel's row permutation from one copy of the matrix and written
+// to the other; finished rows of U go to a third.  Back substitution stages U rows in
+// LDS; the running differences stay sequential in the reference's order.
 __global__ __launch_bounds__(VLM_SOLVE_THREADS) void k_vlm_solve(VlmBatch B) {
-    __shared__ double col[2][AERO_VLM_MAX_HORSESHOES];
-    __shared__ double row[AERO_VLM_MAX_HORSESHOES];
+    __shared__ double P[VLM_PANEL_CAP];                 // panel, rows K .. n-1, stride w
     __shared__ double rhs[AERO_VLM_MAX_HORSESHOES];
     __shared__ double x[AERO_VLM_MAX_HORSESHOES];
+    __shared__ double prod[AERO_VLM_MAX_HORSESHOES];
+    __shared__ int rowmap[AERO_VLM_MAX_HORSESHOES];    // position -> row of the input copy
+    __shared__ uint8_t allnz[AERO_VLM_MAX_HORSESHOES];  // no zero multiplier in this panel
     __shared__ double red_v[VLM_SOLVE_WAVES];
     __shared__ int red_i[VLM_SOLVE_WAVES];
-    __shared__ int piv_s;
+    __shared__ double d_sh;
     const uint32_t l = blockIdx.x;
     const uint32_t base = B.off[l];
     const int n = (int)(B.off[l + 1] - base);
-    double* A = B.A + B.moff[l];
+    if (n == 0) return;
+    double* Ain = B.A + B.moff[l];
+    double* Aout = B.A2 + B.moff[l];
+    double* U = B.U + B.moff[l];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int i = tid; i < n; i += VLM_SOLVE_THREADS) {
-        rhs[i] = B.rhs[base + i];
-        col[0][i] = A[(size_t)i * n];
-    }
-    __syncthreads();
-    int cur = 0;
-    for (int k = 0; k < n; k++) {
-        // pivot: the first row of largest |A[i][k]|, NaNs never taken, and no move at
-        // all when |A[k][k]| is NaN (the reference's strict > scan)
-        double bv = -1.0;
-        int bi = n;
-        for (int i = k + tid; i < n; i += VLM_SOLVE_THREADS) {
-            const double v = __builtin_fabs(col[cur][i]);
-            if (v > bv) {
-                bv = v;
-                bi = i;
-            }
+    const int nb = n >= VLM_PANEL_CAP / 16 ? (VLM_PANEL_CAP / n < 1 ? 1 : VLM_PANEL_CAP / n) : 16;
+    for (int i = tid; i < n; i += VLM_SOLVE_THREADS) rhs[i] = B.rhs[base + i];
+    for (int K = 0; K < n; K += nb) {
+        const int w = n - K < nb ? n - K : nb;
+        const int rows = n - K;
+        for (int e = tid; e < rows * w; e += VLM_SOLVE_THREADS) {
+            const int r = e / w, c = e - r * w;
+            P[e] = Ain[(size_t)(K + r) * n + K + c];
         }
-        for (int s = 32; s > 0; s >>= 1) {
-            const double ov = __shfl_xor(bv, s);
-            const int oi = __shfl_xor(bi, s);
-            if (ov > bv || (ov == bv && oi < bi)) {
-                bv = ov;
-                bi = oi;
-            }
-        }
-        if (lane == 0) {
-            red_v[wave] = bv;
-            red_i[wave] = bi;
+        for (int i = K + tid; i < n; i += VLM_SOLVE_THREADS) {
+            rowmap[i] = i;
+            allnz[i] = 1;
         }
         __syncthreads();
-        if (tid == 0) {
-            double v = red_v[0];
-            int p = red_i[0];
-            for (int w = 1; w < VLM_SOLVE_WAVES; w++)
-                if (red_v[w] > v || (red_v[w] == v && red_i[w] < p)) {
-                    v = red_v[w];
-                    p = red_i[w];
+        if (K + wave * 64 < n) {  // pivot candidates of the panel's first column
+            const int i = K + tid;  // rows <= 1024: one per thread
+            double v = -1.0;
+            int bi = 0x7fffffff;
+            if (i < n) {
+                const double a = __builtin_fabs(P[(i - K) * w]);
+                if (a == a) {
+                    v = a;
+                    bi = i;
                 }
-            const double akk = col[cur][k];
-            piv_s = (akk != akk || p >= n) ? k : p;
-        }
-        __syncthreads();
-        const int piv = piv_s;
-        // row swap (columns below the diagonal are never read again) and the pivot row
-        double* rk = A + (size_t)k * n;
-        double* rp = A + (size_t)piv * n;
-        for (int j = k + tid; j < n; j += VLM_SOLVE_THREADS) {
-            const double b = rp[j];
-            if (piv != k) rp[j] = rk[j];
-            rk[j] = b;
-            row[j] = b;
-        }
-        if (tid == 0 && piv != k) {
-            const double t = rhs[k];
-            rhs[k] = rhs[piv];
-            rhs[piv] = t;
-            const double c = col[cur][k];
-            col[cur][k] = col[cur][piv];
-            col[cur][piv] = c;
-        }
-        __syncthreads();
-        double d = row[k];
-        if (d == 0.0 || d != d) d = 1e-12;
-        const int nxt = cur ^ 1;
-        for (int i = k + 1 + wave; i < n; i += VLM_SOLVE_WAVES) {
-            const double f = col[cur][i] / d;
-            double* ri = A + (size_t)i * n;
-            if (f == 0.0) {
-                if (lane == 0 && k + 1 < n) col[nxt][i] = ri[k + 1];
-                continue;
             }
-            for (int j = k + 1 + lane; j < n; j += 64) {
-                const double v = ri[j] - f * row[j];
-                ri[j] = v;
-                if (j == k + 1) col[nxt][i] = v;
+            vlm_argmax(v, bi);
+            if (lane == 0) {
+                red_v[wave] = v;
+                red_i[wave] = bi;
             }
-            if (lane == 0) rhs[i] = rhs[i] - f * rhs[k];
+        } else if (lane == 0) {
+            red_v[wave] = -1.0;
+            red_i[wave] = 0x7fffffff;
         }
         __syncthreads();
-        cur = nxt;
+        for (int c = 0; c < w; c++) {
+            const int m = K + c;
+            if (wave == 0) {
+                // the pivot (the reference's strict > scan from row m: the first largest
+                // |A[i][m]|, and no move when |A[m][m]| is NaN), the row swap, the pivot value
+                double v = lane < VLM_SOLVE_WAVES ? red_v[lane] : -1.0;
+                int bi = lane < VLM_SOLVE_WAVES ? red_i[lane] : 0x7fffffff;
+                vlm_argmax_row16(v, bi);  // the 16 wave results sit in lanes 0 .. 15
+                const double amm = P[c * w + c];
+                const int piv = (amm != amm || bi >= n || bi < m) ? m : bi;
+                if (piv != m) {
+                    if (lane < w) {
+                        const double t = P[c * w + lane];
+                        P[c * w + lane] = P[(piv - K) * w + lane];
+                        P[(piv - K) * w + lane] = t;
+                    }
+                    if (lane == 0) {
+                        const double t = rhs[m];
+                        rhs[m] = rhs[piv];
+                        rhs[piv] = t;
+                        const int r = rowmap[m];
+                        rowmap[m] = rowmap[piv];
+                        rowmap[piv] = r;
+                        const uint8_t z = allnz[m];
+                        allnz[m] = allnz[piv];
+                        allnz[piv] = z;
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (lane == 0) {
+                    double d = P[c * w + c];
+                    if (d == 0.0 || d != d) d = 1e-12;
+                    d_sh = d;
+                }
+            }
+            __syncthreads();
+            if (m + 1 + wave * 64 < n) {  // eliminate below the pivot; candidates for the next column
+                const int i = m + 1 + tid;
+                double v = -1.0;
+                int bi = 0x7fffffff;
+                if (i < n) {
+                    double* pi = P + (i - K) * w;
+                    const double* pm = P + c * w;
+                    const double f = pi[c] / d_sh;
+                    pi[c] = f;  // the multiplier, kept for the columns right of the panel
+                    if (f == 0.0) allnz[i] = 0;
+                    if (f != 0.0) {
+                        // pivot-row entries first, eight at a time, so their loads run ahead
+#pragma unroll
+                        for (int q0 = 0; q0 < VLM_PANEL_W; q0 += 8) {
+                            double b[8];
+#pragma unroll
+                            for (int q = 0; q < 8; q++)
+                                if (q0 + q > c && q0 + q < w) b[q] = pm[q0 + q];
+#pragma unroll
+                            for (int q = 0; q < 8; q++)
+                                if (q0 + q > c && q0 + q < w) pi[q0 + q] = pi[q0 + q] - f * b[q];
+                        }
+                        rhs[i] = rhs[i] - f * rhs[m];
+                    }
+                    if (c + 1 < w) {
+                        const double a = __builtin_fabs(pi[c + 1]);
+                        if (a == a) {
+                            v = a;
+                            bi = i;
+                        }
+                    }
+                }
+                vlm_argmax(v, bi);
+                if (lane == 0) {
+                    red_v[wave] = v;
+                    red_i[wave] = bi;
+                }
+            } else if (lane == 0) {
+                red_v[wave] = -1.0;
+                red_i[wave] = 0x7fffffff;
+            }
+            __syncthreads();
+        }
+        // the panel's rows of U (at and right of the diagonal, within the panel)
+        for (int e = tid; e < w * w; e += VLM_SOLVE_THREADS) {
+            const int r = e / w, c = e - r * w;
+            if (c >= r) U[(size_t)(K + r) * n + K + c] = P[e];
+        }
+        // columns right of the panel: the panel rows' U entries (the panel's steps in
+        // order on each), then every later row
+        const int C = n - K - w;
+        if (C > 0) {
+            const int G = VLM_SOLVE_THREADS / C < 1 ? 1 : VLM_SOLVE_THREADS / C;
+            if (tid < G * C) {
+                const int col = K + w + tid % C, g = tid / C;
+                double u[VLM_PANEL_W];
+#pragma unroll
+                for (int c = 0; c < VLM_PANEL_W; c++)
+                    if (c < w) u[c] = Ain[(size_t)rowmap[K + c] * n + col];
+#pragma unroll
+                for (int c = 1; c < VLM_PANEL_W; c++) {
+                    if (c < w) {
+                        double v = u[c];
+#pragma unroll
+                        for (int p = 0; p < VLM_PANEL_W - 1; p++) {
+                            if (p < c) {
+                                const double f = P[c * w + p];
+                                if (f != 0.0) v = v - f * u[p];
+                            }
+                        }
+                        u[c] = v;
+                    }
+                }
+                if (g == 0)
+#pragma unroll
+                    for (int c = 0; c < VLM_PANEL_W; c++)
+                        if (c < w) U[(size_t)(K + c) * n + col] = u[c];
+                for (int i0 = K + w + g; i0 < n; i0 += VLM_ROW_BATCH * G) {
+                    double v[VLM_ROW_BATCH];  // a batch of rows in flight at once
+#pragma unroll
+                    for (int q = 0; q < VLM_ROW_BATCH; q++) {
+                        const int i = i0 + q * G;
+                        if (i < n) v[q] = Ain[(size_t)rowmap[i] * n + col];
+                    }
+#pragma unroll
+                    for (int q = 0; q < VLM_ROW_BATCH; q++) {
+                        const int i = i0 + q * G;
+                        if (i < n) {
+                            const double* pi = P + (i - K) * w;
+                            double a = v[q];
+                            if (allnz[i]) {  // no step skips this row: every update applies
+#pragma unroll
+                                for (int p = 0; p < VLM_PANEL_W; p++)
+                                    if (p < w) a = a - pi[p] * u[p];
+                            } else {
+#pragma unroll
+                                for (int p = 0; p < VLM_PANEL_W; p++) {
+                                    if (p < w) {
+                                        const double f = pi[p];
+                                        if (f != 0.0) a = a - f * u[p];
+                                    }
+                                }
+                            }
+                            Aout[(size_t)i * n + col] = a;
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        double* t = Ain;
+        Ain = Aout;
+        Aout = t;
     }
-    // back substitution: products in parallel, the running difference in order
-    for (int i = n - 1; i >= 0; i--) {
-        const double* ri = A + (size_t)i * n;
-        for (int j = i + 1 + tid; j < n; j += VLM_SOLVE_THREADS) row[j] = ri[j] * x[j];
-        __syncthreads();
-        if (tid == 0) {
-            double s = rhs[i];
-            for (int j = i + 1; j < n; j++) s -= row[j];
-            double d = ri[i];
-            if (d == 0.0 || d != d) d = 1e-12;
-            x[i] = s / d;
+    // back substitution: x[i] = (rhs[i] - U[i][i+1] x[i+1] - U[i][i+2] x[i+2] - ...) /
+    // (U[i][i] || 1e-12), the differences taken in that order
+    for (int hi = n - 1; hi >= 0;) {
+        int lo = hi, used = n - hi;
+        while (lo > 0 && used + (n - (lo - 1)) <= VLM_PANEL_CAP) {
+            lo--;
+            used += n - lo;
+        }
+        // row i of the batch at P[off(i)], off(i) = sum over r in (i, hi] of (n - r)
+        for (int i = lo + wave; i <= hi; i += VLM_SOLVE_WAVES) {
+            const int off = (hi - i) * n - ((hi * (hi + 1)) / 2 - (i * (i + 1)) / 2);
+            for (int j = i + lane; j < n; j += 64) P[off + (j - i)] = U[(size_t)i * n + j];
         }
         __syncthreads();
+        if (wave == 0) {
+            for (int i = hi; i >= lo; i--) {
+                const int off = (hi - i) * n - ((hi * (hi + 1)) / 2 - (i * (i + 1)) / 2);
+                for (int j = i + 1 + lane; j < n; j += 64) prod[j] = P[off + (j - i)] * x[j];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (lane == 0) {
+                    double s = rhs[i];
+                    int j = i + 1;
+                    for (; j + 16 <= n; j += 16) {
+                        double p[16];
+#pragma unroll
+                        for (int q = 0; q < 16; q++) p[q] = prod[j + q];
+#pragma unroll
+                        for (int q = 0; q < 16; q++) s -= p[q];
+                    }
+                    for (; j < n; j++) s -= prod[j];
+                    double d = P[off];
+                    if (d == 0.0 || d != d) d = 1e-12;
+                    x[i] = s / d;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+        __syncthreads();
+        hi = lo - 1;
     }
     for (int i = tid; i < n; i += VLM_SOLVE_THREADS) B.gamma[base + i] = x[i];
 }
@@ -292,9 +494,24 @@ __global__ __launch_bounds__(VLM_FORCE_THREADS) void k_vlm_forces(VlmBatch B) {
             seg[3 * s + 2] = o[2];
         }
         __syncthreads();
-        if (tid < 3)
-            for (uint32_t s = 0; s < cnt; s++)
+        if (tid < 3) {
+            // the segment loads run ahead; the additions stay in order
+            uint32_t s = 0;
+            for (; s + 8 <= cnt; s += 8) {
+                double v[8];
+                bool u[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    v[q] = seg[3 * (s + q) + tid];
+                    u[q] = used[s + q];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    if (u[q]) acc += v[q];
+            }
+            for (; s < cnt; s++)
                 if (used[s]) acc += seg[3 * s + tid];
+        }
         __syncthreads();
     }
     if (tid < 3) vel[tid] = acc;
@@ -313,31 +530,57 @@ __global__ __launch_bounds__(VLM_FORCE_THREADS) void k_vlm_forces(VlmBatch B) {
     }
 }
 
-// The sums and outputs of vlm.ts:157-181, one lane per lattice.
-__global__ __launch_bounds__(64) void k_vlm_sum(VlmBatch B) {
-    const uint32_t l = blockIdx.x * 64 + threadIdx.x;
-    if (l >= B.count) return;
+// The sums and outputs of vlm.ts:157-181, one workgroup per lattice: the per-vortex
+// terms staged in LDS, then each sum taken in index order by one lane.
+__global__ __launch_bounds__(256) void k_vlm_sum(VlmBatch B) {
+    __shared__ double st[7][VLM_SUM_CHUNK];
+    __shared__ double acc[7];
+    const uint32_t l = blockIdx.x;
     const uint32_t base = B.off[l], n = B.off[l + 1] - base;
-    double fx = 0.0, fy = 0.0, area = 0.0, mx = 0.0, my = 0.0, cx = 0.0, cy = 0.0;
-    for (uint32_t i = 0; i < n; i++) {
-        const double* t = B.terms + (size_t)5 * (base + i);
-        fx += t[0];
-        fy += t[1];
-        mx += t[3];
-        my += t[4];
-        area += B.hs[(size_t)VLM_REC * (base + i) + 18];
+    const int tid = threadIdx.x;
+    if (tid < 7) acc[tid] = 0.0;  // fx, fy, mx, my, area, cx, cy
+    for (uint32_t i0 = 0; i0 < n; i0 += VLM_SUM_CHUNK) {
+        const uint32_t cnt = n - i0 < (uint32_t)VLM_SUM_CHUNK ? n - i0 : (uint32_t)VLM_SUM_CHUNK;
+        __syncthreads();
+        for (uint32_t r = tid; r < cnt; r += 256) {
+            const double* t = B.terms + (size_t)5 * (base + i0 + r);
+            const double* h = B.hs + (size_t)VLM_REC * (base + i0 + r);
+            st[0][r] = t[0];
+            st[1][r] = t[1];
+            st[2][r] = t[3];
+            st[3][r] = t[4];
+            st[4][r] = h[18];
+            st[5][r] = h[12];
+            st[6][r] = h[13];
+        }
+        __syncthreads();
+        if (tid < 7) {
+            double a = acc[tid];
+            uint32_t r = 0;
+            for (; r + 8 <= cnt; r += 8) {
+                double v[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) v[q] = st[tid][r + q];
+#pragma unroll
+                for (int q = 0; q < 8; q++) a += v[q];
+            }
+            for (; r < cnt; r++) a += st[tid][r];
+            acc[tid] = a;
+        }
     }
-    for (uint32_t i = 0; i < n; i++) cx = cx + B.hs[(size_t)VLM_REC * (base + i) + 12];
-    for (uint32_t i = 0; i < n; i++) cy = cy + B.hs[(size_t)VLM_REC * (base + i) + 13];
-    const double speed = B.speed[l];
-    const double q = ((0.5 * VLM_RHO) * speed) * speed;
-    const double down = -fy;
-    const double nn = n > 1 ? (double)n : 1.0;
-    double* o = B.out + (size_t)4 * l;
-    o[0] = down;
-    o[1] = fx + (VLM_CD0 * q) * area;
-    o[2] = __builtin_fabs(down) > 1.0 ? mx / down : cx / nn;
-    o[3] = __builtin_fabs(fx) > 1.0 ? my / fx : cy / nn;
+    __syncthreads();
+    if (tid == 0) {
+        const double fx = acc[0], fy = acc[1], mx = acc[2], my = acc[3], area = acc[4], cx = acc[5], cy = acc[6];
+        const double speed = B.speed[l];
+        const double q = ((0.5 * VLM_RHO) * speed) * speed;
+        const double down = -fy;
+        const double nn = n > 1 ? (double)n : 1.0;
+        double* o = B.out + (size_t)4 * l;
+        o[0] = down;
+        o[1] = fx + (VLM_CD0 * q) * area;
+        o[2] = __builtin_fabs(down) > 1.0 ? mx / down : cx / nn;
+        o[3] = __builtin_fabs(fx) > 1.0 ? my / fx : cy / nn;
+    }
 }
 
 }  // namespace aero
@@ -382,7 +625,7 @@ extern "C" int aero_vlm_loads(const double* hs, const uint32_t* offsets, uint32_
     if (e == hipSuccess) e = hipMalloc(&d_off, (size_t)(count + 1) * 4);
     if (e == hipSuccess) e = hipMalloc(&d_moff, (size_t)count * 8);
     if (e == hipSuccess) e = hipMalloc(&d_speed, (size_t)count * 8);
-    if (e == hipSuccess && msize) e = hipMalloc(&d_A, msize * sizeof(double));
+    if (e == hipSuccess && msize) e = hipMalloc(&d_A, 3 * msize * sizeof(double));
     if (e == hipSuccess) e = hipMalloc(&d_rhs, vb);
     if (e == hipSuccess) e = hipMalloc(&d_gamma, vb);
     if (e == hipSuccess) e = hipMalloc(&d_terms, 5 * vb);
@@ -396,6 +639,8 @@ extern "C" int aero_vlm_loads(const double* hs, const uint32_t* offsets, uint32_
     B.moff = d_moff;
     B.speed = d_speed;
     B.A = d_A;
+    B.A2 = d_A ? d_A + msize : nullptr;
+    B.U = d_A ? d_A + 2 * msize : nullptr;
     B.rhs = d_rhs;
     B.gamma = d_gamma;
     B.terms = d_terms;
@@ -414,7 +659,7 @@ extern "C" int aero_vlm_loads(const double* hs, const uint32_t* offsets, uint32_
         }
     }
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_vlm_sum, dim3((count + 63) / 64), dim3(64), 0, st, B);
+        hipLaunchKernelGGL(k_vlm_sum, dim3(count), dim3(256), 0, st, B);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, (size_t)count * 4 * sizeof(double), hipMemcpyDeviceToHost, st);
