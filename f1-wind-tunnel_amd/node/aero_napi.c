@@ -811,6 +811,48 @@ static napi_value js_bake_sdf(napi_env env, napi_callback_info info) {
     return arr;
 }
 
+/* vlmLoads(hs: Float64Array[20n], offsets: Uint32Array[count + 1], speeds: Float64Array[count],
+ *          wantGamma) -> { loads: Float64Array[4 count], gamma: Float64Array[n] | null }
+ * (wingLoads of src/engine/vlm.ts:140-182 for every lattice; loads holds downforceN, dragN,
+ * copX, dragY per lattice) */
+static napi_value js_vlm_loads(napi_env env, napi_callback_info info) {
+    napi_value a[4];
+    if (!get_args(env, info, 4, a)) return NULL;
+    size_t nh = 0, no = 0, ns = 0;
+    const double* hs = (const double*)typed(env, a[0], napi_float64_array, &nh);
+    if (!hs) return NULL;
+    const uint32_t* off = (const uint32_t*)typed(env, a[1], napi_uint32_array, &no);
+    if (!off) return NULL;
+    const double* sp = (const double*)typed(env, a[2], napi_float64_array, &ns);
+    if (!sp) return NULL;
+    bool want = false;
+    NAPI_CALL(env, napi_get_value_bool(env, a[3], &want));
+    if (no < 2 || ns != no - 1 || nh % 20 || off[no - 1] != nh / 20) {
+        napi_throw_range_error(env, "AERO_ARGS",
+                               "offsets needs count + 1 entries ending at the horseshoe count, one speed per lattice");
+        return NULL;
+    }
+    const uint32_t count = (uint32_t)(no - 1);
+    void* po = NULL;
+    napi_value loads = make_typed(env, napi_float64_array, (size_t)count * 4, 8, &po);
+    if (!loads) return NULL;
+    void* pg = NULL;
+    napi_value gamma;
+    if (want) {
+        gamma = make_typed(env, napi_float64_array, nh / 20 ? nh / 20 : 1, 8, &pg);
+        if (!gamma) return NULL;
+    } else {
+        NAPI_CALL(env, napi_get_null(env, &gamma));
+    }
+    const int rc = aero_vlm_loads(nh ? hs : NULL, off, count, sp, (double*)po, (double*)pg);
+    if (rc) return throw_rc(env, rc, NULL, "aero_vlm_loads");
+    napi_value obj;
+    NAPI_CALL(env, napi_create_object(env, &obj));
+    NAPI_CALL(env, napi_set_named_property(env, obj, "loads", loads));
+    NAPI_CALL(env, napi_set_named_property(env, obj, "gamma", gamma));
+    return obj;
+}
+
 static napi_value js_version(napi_env env, napi_callback_info info) {
     (void)info;
     napi_value v;
@@ -838,6 +880,7 @@ static napi_value init_module(napi_env env, napi_value exports) {
         {"batchDestroy", js_batch_destroy}, {"batchStep", js_batch_step},
         {"batchRunGraph", js_batch_run_graph}, {"batchAutotune", js_batch_autotune},
         {"batchSetLaunchShape", js_batch_set_launch_shape}, {"bakeSdf", js_bake_sdf},
+        {"vlmLoads", js_vlm_loads},
     };
     for (size_t i = 0; i < sizeof fns / sizeof fns[0]; i++) {
         napi_value f;

@@ -103,9 +103,50 @@ export declare function aeroMap(opts: {
   prims?: (front: number, rear: number) => Uint8Array | null; settle: number; measure: number;
   front?: number[]; rear?: number[]; storage?: number; stepsPerGraph?: number; launchShape?: number;
   uLb?: number; tau0?: number; smag?: number; kmh?: number;
+  /** W17 wings: each cell's bal is frontAxleShare of the panel-method loads at its ride heights. */
+  wings?: { rig: RigData; setup: W17WingSetup; speed: number };
 }): { cells: AeroMapCell[]; deviceMs: number };
 
 /** bakeGrid of scripts/bake-car.ts:237-316 on the GPU (aero_bake_sdf): f16 words, x fastest. */
 export declare function bakeGrid(tris: Float64Array, layout: { origin: [number, number, number]; h: number;
                                  dims: [number, number, number] }, band: number, closing: number,
                                  shell: number): Uint16Array;
+
+// ---------- Wing loads (vortex lattice, src/engine/vlm.ts) ----------
+type Pt = [number, number];
+type V3 = [number, number, number];
+export interface WingStation { z: number; le: Pt; mid: Pt; te: Pt }
+export interface RigData {
+  frontHinges: Pt[][]; rearMainHinge: Pt; rearFlapHinge: Pt;
+  wings: { front: WingStation[][]; rear: WingStation[][] };
+}
+export interface WingPose { rotate: (element: number, z: number, p: Pt) => Pt; toWorld: (p: V3) => V3 }
+export interface WingLoads { downforceN: number; dragN: number; copX: number; dragY: number }
+export interface W17WingSetup {
+  frontDeg: number; rearFlapDeg: number; rearMainDeg: number; rideH: number; rearRideH: number;
+  gurney: boolean; frontOn: boolean; rearOn: boolean; rearTwistDeg: number; flapScale: number;
+}
+export declare const FRONT_SPAN: [number, number];
+export declare const REAR_SPAN: [number, number];
+export declare const NO_LOAD: WingLoads;
+/** buildLattice (vlm.ts:86-113): 20 doubles per horseshoe (a, b, ta, tb, cp, n, area, 0). */
+export declare function buildLattice(elements: WingStation[][], pose: WingPose, span: [number, number],
+                                     spanStrips?: number, chordPanels?: number): Float64Array;
+/** aero_vlm_loads: every lattice in one GPU call. */
+export declare function solveLattices(lattices: Float64Array[], speeds: number[], wantGamma?: boolean):
+  { loads: WingLoads[]; gamma: Float64Array[] | null };
+/** wingLoads (vlm.ts:140-182). */
+export declare function wingLoads(elements: WingStation[][], pose: WingPose, span: [number, number], speed: number,
+                                  spanStrips?: number, chordPanels?: number): WingLoads;
+/** withGurney (vlm.ts:207-221). */
+export declare function withGurney(elements: WingStation[][]): WingStation[][];
+export declare function w17Poses(rig: RigData, setup: W17WingSetup): { front: WingPose; rear: WingPose };
+export declare function w17Lattices(rig: RigData, setup: W17WingSetup):
+  { front: Float64Array | null; rear: Float64Array | null };
+/** w17WingLoads (vlm.ts:234-271), the vlm-worker's reply. */
+export declare function w17WingLoads(rig: RigData, setup: W17WingSetup, speed: number):
+  { front: WingLoads; rear: WingLoads };
+export declare function w17WingLoadsMany(rig: RigData, setups: W17WingSetup[], speeds: number[]):
+  { front: WingLoads; rear: WingLoads }[];
+/** frontAxleShare (main.ts:618-625). */
+export declare function frontAxleShare(loads: { front: WingLoads; rear: WingLoads }): number | undefined;

@@ -418,9 +418,20 @@ function aeroMap(opts) {
     let ms = batch.runGraph(opts.settle, opts.stepsPerGraph);
     for (const e of batch.engines) e.readForces(true);
     ms += batch.runGraph(opts.measure, opts.stepsPerGraph);
+    // bal: the front-axle share of the panel-method wing load at each pair when the W17
+    // wings are given (opts.wings = { rig, setup, speed }), else 0.5 (aeromap.ts:101)
+    const bal = pairs.map(() => 0.5);
+    if (opts.wings) {
+      const w = opts.wings;
+      const setups = pairs.map(([f, r]) => Object.assign({}, w.setup, { rideH: f, rearRideH: r }));
+      w17WingLoadsMany(w.rig, setups, setups.map(() => w.speed)).forEach((ld, i) => {
+        const share = frontAxleShare(ld);
+        if (share !== undefined) bal[i] = share;
+      });
+    }
     const cells = pairs.map(([f, r], i) => {
       const F = batch.engines[i].readForces(true);
-      return { front: f, rear: r, down: F.downforceN / KGF, eff: F.downforceN / Math.max(F.dragN, 1), bal: 0.5,
+      return { front: f, rear: r, down: F.downforceN / KGF, eff: F.downforceN / Math.max(F.dragN, 1), bal: bal[i],
                forces: F };
     });
     return { cells, deviceMs: ms };
@@ -437,5 +448,197 @@ function bakeGrid(tris, layout, band, closing, shell) {
                        shell);
 }
 
-module.exports = { HipAeroEngine, HipAeroBatch, aeroMap, bakeGrid, AEROMAP_FRONT, AEROMAP_REAR, ShapeKind, Storage, FlowMode, ColorField, packShape, packPSim, packStream, viewUniform,
+// ---------- Wing loads (vortex lattice) ----------
+// The W17 panel method of src/engine/vlm.ts: the lattice is built here (FP64, the
+// reference's operations in its order, V8's own Math functions), the influence matrix,
+// the solve and the Kutta-Joukowski forces run on the GPU (aero_vlm_loads), any number of
+// lattices per call.  A horseshoe is 20 doubles: a, b, ta, tb, cp, n, area, 0.
+
+const VLM_REC = 20;
+const FRONT_SPAN = [0.09, 0.62]; // vlm.ts:32-33
+const REAR_SPAN = [0.0, 0.53];
+const GURNEY_HEIGHT = 0.004; // vlm.ts:204
+const VLM_RIG = { frontAxleX: 1.0, baseRideH: 0.092, rearAxleX: 4.4, dz: 0.01, bins: 71,
+                  fadeIn: [0.16, 0.22], fadeOut: [0.5, 0.56], frontThinFactor: 0.65 }; // car-model.ts:15-46
+const NO_LOAD = Object.freeze({ downforceN: 0, dragN: 0, copX: 0, dragY: 0 });
+const vlerp = (a, b, t) => a + (b - a) * t;
+
+/** station (vlm.ts:68-77): the wing section at |z|. */
+function wingStation(stations, z) {
+  if (z <= stations[0].z) return stations[0];
+  const last = stations[stations.length - 1];
+  if (z >= last.z) return last;
+  let i = 0;
+  while (stations[i + 1].z < z) i++;
+  const s0 = stations[i], s1 = stations[i + 1];
+  const t = (z - s0.z) / (s1.z - s0.z);
+  const mix = (p, q) => [vlerp(p[0], q[0], t), vlerp(p[1], q[1], t)];
+  return { z, le: mix(s0.le, s1.le), mid: mix(s0.mid, s1.mid), te: mix(s0.te, s1.te) };
+}
+
+/** camberPoint (vlm.ts:80-84): the quadratic Bezier through le, mid, te. */
+function camberPoint(s, t) {
+  const c0 = 2 * s.mid[0] - 0.5 * (s.le[0] + s.te[0]), c1 = 2 * s.mid[1] - 0.5 * (s.le[1] + s.te[1]);
+  const u = 1 - t;
+  return [u * u * s.le[0] + 2 * u * t * c0 + t * t * s.te[0], u * u * s.le[1] + 2 * u * t * c1 + t * t * s.te[1]];
+}
+
+/** buildLattice (vlm.ts:86-113) into a Float64Array of 20-double horseshoes. */
+function buildLattice(elements, pose, span, spanStrips = 10, chordPanels = 4) {
+  const out = new Float64Array(elements.length * 2 * spanStrips * chordPanels * VLM_REC);
+  let o = 0;
+  const put = (v) => { out[o++] = v[0]; out[o++] = v[1]; out[o++] = v[2]; };
+  elements.forEach((stations, e) => {
+    const z0 = Math.min(stations[0].z, span[0]), z1 = Math.max(stations[stations.length - 1].z, span[1]);
+    const point = (z, t, side) => {
+      const local = pose.rotate(e, z, camberPoint(wingStation(stations, z), t));
+      return pose.toWorld([local[0], local[1], z * side]);
+    };
+    for (const side of [1, -1]) {
+      for (let i = 0; i < spanStrips; i++) {
+        const za = vlerp(z0, z1, i / spanStrips), zb = vlerp(z0, z1, (i + 1) / spanStrips);
+        const zl = side > 0 ? za : zb, zr = side > 0 ? zb : za;
+        for (let j = 0; j < chordPanels; j++) {
+          const t0 = j / chordPanels, t1 = (j + 1) / chordPanels;
+          const tq = vlerp(t0, t1, 0.25), tc = vlerp(t0, t1, 0.75);
+          const p00 = point(zl, t0, side), p01 = point(zr, t0, side), p10 = point(zl, t1, side), p11 = point(zr, t1, side);
+          const u = [p11[0] - p00[0], p11[1] - p00[1], p11[2] - p00[2]];
+          const w = [p01[0] - p10[0], p01[1] - p10[1], p01[2] - p10[2]];
+          const n = [u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]];
+          const len = Math.hypot(n[0], n[1], n[2]) || 1;
+          put(point(zl, tq, side));
+          put(point(zr, tq, side));
+          put(point(zl, 1, side));
+          put(point(zr, 1, side));
+          put(point((zl + zr) / 2, tc, side));
+          put([n[0] / len, n[1] / len, n[2] / len]);
+          out[o++] = len / 2;
+          out[o++] = 0;
+        }
+      }
+    }
+  });
+  return out;
+}
+
+/** aero_vlm_loads over many lattices (Float64Arrays of horseshoes) in one GPU call:
+ *  { loads: WingLoads[], gamma: Float64Array[] | null }. */
+function solveLattices(lattices, speeds, wantGamma = false) {
+  if (lattices.length !== speeds.length) throw new RangeError("one speed per lattice");
+  if (lattices.length === 0) return { loads: [], gamma: wantGamma ? [] : null };
+  const offsets = new Uint32Array(lattices.length + 1);
+  lattices.forEach((h, i) => { offsets[i + 1] = offsets[i] + h.length / VLM_REC; });
+  const hs = new Float64Array(offsets[lattices.length] * VLM_REC);
+  lattices.forEach((h, i) => hs.set(h, offsets[i] * VLM_REC));
+  const r = addon.vlmLoads(hs, offsets, Float64Array.from(speeds), wantGamma);
+  const loads = lattices.map((_, i) => ({ downforceN: r.loads[4 * i], dragN: r.loads[4 * i + 1],
+                                          copX: r.loads[4 * i + 2], dragY: r.loads[4 * i + 3] }));
+  const gamma = wantGamma ? lattices.map((_, i) => r.gamma.slice(offsets[i], offsets[i + 1])) : null;
+  return { loads, gamma };
+}
+
+/** wingLoads (vlm.ts:140-182) of one wing. */
+function wingLoads(elements, pose, span, speed, spanStrips = 10, chordPanels = 4) {
+  return solveLattices([buildLattice(elements, pose, span, spanStrips, chordPanels)], [speed]).loads[0];
+}
+
+/** withGurney (vlm.ts:207-221): the tab at the last element's trailing edge. */
+function withGurney(elements) {
+  const tab = elements[elements.length - 1].map((s) => {
+    const dx = s.te[0] - s.le[0], dy = s.te[1] - s.le[1];
+    const l = Math.hypot(dx, dy) || 1;
+    const up = [-dy / l, dx / l];
+    return { z: s.z, le: s.te, mid: [s.te[0] + up[0] * GURNEY_HEIGHT * 0.5, s.te[1] + up[1] * GURNEY_HEIGHT * 0.5],
+             te: [s.te[0] + up[0] * GURNEY_HEIGHT, s.te[1] + up[1] * GURNEY_HEIGHT] };
+  });
+  return elements.concat([tab]);
+}
+
+const vsmooth = (a, b, x) => {
+  const t = Math.min(1, Math.max(0, (x - a) / (b - a)));
+  return t * t * (3 - 2 * t);
+};
+const rotateAbout = (p, h, a) => {
+  if (a === 0) return p;
+  const c = Math.cos(a), s = Math.sin(a), dx = p[0] - h[0], dy = p[1] - h[1];
+  return [h[0] + c * dx - s * dy, h[1] + s * dx + c * dy];
+};
+const scaleAbout = (p, h, k) => (k === 1 ? p : [h[0] + (p[0] - h[0]) * k, h[1] + (p[1] - h[1]) * k]);
+
+/** The front and rear wing poses of w17WingLoads (vlm.ts:235-265). */
+function w17Poses(rig, setup) {
+  const R = VLM_RIG, DEG = Math.PI / 180;
+  const rake = Math.atan((setup.rearRideH - setup.rideH) / (R.rearAxleX - R.frontAxleX));
+  const c = Math.cos(rake), s = Math.sin(rake);
+  const toWorld = (p) => {
+    const bx = p[0] - R.frontAxleX, by = p[1] - R.baseRideH;
+    return [R.frontAxleX + c * bx - s * by, setup.rideH + s * bx + c * by, p[2]];
+  };
+  const hinge = (table, z) => {
+    const f = Math.min(Math.max(z / R.dz, 0), R.bins - 1);
+    const i = Math.floor(f), j = Math.min(i + 1, R.bins - 1), t = f - i;
+    return [vlerp(table[i][0], table[j][0], t), vlerp(table[i][1], table[j][1], t)];
+  };
+  const frontWeight = (z) => vsmooth(R.fadeIn[0], R.fadeIn[1], z) * (1 - vsmooth(R.fadeOut[0], R.fadeOut[1], z));
+  const front = {
+    toWorld,
+    rotate: (e, z, p) => {
+      if (e === 0) return p;
+      const h = hinge(rig.frontHinges[e - 1], z);
+      const k = 1 - (1 - setup.flapScale) * R.frontThinFactor;
+      return rotateAbout(scaleAbout(p, h, k), h, setup.frontDeg * DEG * (e === 1 ? 0.8 : 1) * frontWeight(z));
+    },
+  };
+  const rear = {
+    toWorld,
+    rotate: (e, z, p) => {
+      const twist = setup.rearTwistDeg * vsmooth(0.12, 0.48, z) * DEG;
+      if (e === 0) return rotateAbout(p, rig.rearMainHinge, setup.rearMainDeg * DEG + twist);
+      return rotateAbout(scaleAbout(p, rig.rearFlapHinge, setup.flapScale), rig.rearFlapHinge,
+                         setup.rearFlapDeg * DEG + twist);
+    },
+  };
+  return { front, rear };
+}
+
+/** The lattices w17WingLoads solves: { front, rear }, null for a wing that is off. */
+function w17Lattices(rig, setup) {
+  const { front, rear } = w17Poses(rig, setup);
+  const rearElements = setup.gurney ? withGurney(rig.wings.rear) : rig.wings.rear;
+  return { front: setup.frontOn ? buildLattice(rig.wings.front, front, FRONT_SPAN) : null,
+           rear: setup.rearOn ? buildLattice(rearElements, rear, REAR_SPAN) : null };
+}
+
+/** w17WingLoads (vlm.ts:234-271) for many setups: every lattice in one GPU call. */
+function w17WingLoadsMany(rig, setups, speeds) {
+  const lattices = [], sp = [], slots = [];
+  setups.forEach((st, k) => {
+    const l = w17Lattices(rig, st);
+    for (const name of ["front", "rear"]) {
+      if (l[name]) { slots.push([k, name]); lattices.push(l[name]); sp.push(speeds[k]); }
+    }
+  });
+  const res = setups.map(() => ({ front: NO_LOAD, rear: NO_LOAD }));
+  const { loads } = solveLattices(lattices, sp);
+  slots.forEach(([k, name], i) => { res[k][name] = loads[i]; });
+  return res;
+}
+
+/** w17WingLoads (vlm.ts:234-271): { front, rear } WingLoads, the vlm-worker's reply. */
+function w17WingLoads(rig, setup, speed) {
+  return w17WingLoadsMany(rig, [setup], [speed])[0];
+}
+
+/** frontAxleShare (main.ts:618-625): the front axle's share of the wing downforce;
+ *  undefined below 100 N in total. */
+function frontAxleShare(loads) {
+  const wb = VLM_RIG.rearAxleX - VLM_RIG.frontAxleX;
+  const wings = [loads.front, loads.rear];
+  const total = wings.reduce((a, w) => a + w.downforceN, 0);
+  if (total < 100) return undefined;
+  return wings.reduce((a, w) => a + w.downforceN * (VLM_RIG.rearAxleX - w.copX) / wb, 0) / total;
+}
+
+module.exports = { HipAeroEngine, HipAeroBatch, aeroMap, bakeGrid, buildLattice, solveLattices, wingLoads, withGurney,
+                   w17Poses, w17Lattices, w17WingLoads, w17WingLoadsMany, frontAxleShare, FRONT_SPAN, REAR_SPAN, NO_LOAD, AEROMAP_FRONT, AEROMAP_REAR, ShapeKind, Storage, FlowMode, ColorField, packShape, packPSim, packStream, viewUniform,
                    addon, QUALITIES, PARTICLES, MAX_LINES, STREAM_STEPS, LATTICE_U, TAU0, SMAG };

@@ -118,3 +118,51 @@ fs.writeFileSync({json.dumps(str(op))}, Buffer.from(out.buffer, out.byteOffset, 
         got = np.fromfile(op, dtype=np.uint16)
     assert np.array_equal(got, want)
 
+
+
+def test_node_w17_wing_loads_match_oracle():
+    """w17WingLoadsMany and solveLattices (hip-engine.js over aero_vlm_loads) on lattices
+    the JavaScript builder made: bit-identical to the FP64 oracle on the same lattices;
+    the front-axle share the aero map reports."""
+    import tempfile
+
+    from oracle import oracle as orc
+
+    rig_path = ROOT / "tests" / "golden" / "w17_rig.json"
+    with tempfile.TemporaryDirectory() as tmp:
+        lp, op = Path(tmp) / "lat.f64", Path(tmp) / "out.json"
+        script = f"""
+const fs = require('fs');
+const {{ w17Lattices, w17WingLoadsMany, solveLattices, frontAxleShare }} = require('./hip-engine');
+const rig = JSON.parse(fs.readFileSync({json.dumps(str(rig_path))}));
+const setups = [{{ frontDeg: 0, rearFlapDeg: 0, rearMainDeg: 0, rideH: 0.12, rearRideH: 0.14, gurney: false,
+                  frontOn: true, rearOn: true, rearTwistDeg: 0, flapScale: 1 }},
+                {{ frontDeg: 7, rearFlapDeg: 9, rearMainDeg: 3, rideH: 0.09, rearRideH: 0.2, gurney: true,
+                  frontOn: true, rearOn: true, rearTwistDeg: -16, flapScale: 0.72 }}];
+const speed = 250 / 3.6;
+const lats = [];
+for (const s of setups) {{ const l = w17Lattices(rig, s); lats.push(l.front, l.rear); }}
+const r = solveLattices(lats, lats.map(() => speed), true);
+const many = w17WingLoadsMany(rig, setups, [speed, speed]);
+const o = new Float64Array(lats.reduce((a, p) => a + p.length, 0));
+let k = 0;
+for (const p of lats) {{ o.set(p, k); k += p.length; }}
+fs.writeFileSync({json.dumps(str(lp))}, Buffer.from(o.buffer));
+fs.writeFileSync({json.dumps(str(op))}, JSON.stringify({{ sizes: lats.map((p) => p.length / 20), loads: r.loads,
+  gamma: r.gamma.map((g) => Array.from(g)), many, share: many.map(frontAxleShare) }}));
+"""
+        r = subprocess.run(["node", "-e", script], cwd=NODE_DIR, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        flat = np.fromfile(lp, np.float64).reshape(-1, 20)
+        res = json.loads(op.read_text())
+    vo = orc.VLMOracle()
+    off = np.concatenate([[0], np.cumsum(res["sizes"])])
+    keys = ("downforceN", "dragN", "copX", "dragY")
+    for k in range(len(res["sizes"])):
+        ref, g = vo.loads(flat[off[k]:off[k + 1]], 250 / 3.6)
+        got = [res["loads"][k][n] for n in keys]
+        assert np.array_equal(np.array(got).view(np.uint64), ref.view(np.uint64)), k
+        assert np.array_equal(np.array(res["gamma"][k]).view(np.uint64), g.view(np.uint64)), k
+        wing = res["many"][k // 2]["front" if k % 2 == 0 else "rear"]
+        assert [wing[n] for n in keys] == got
+    assert all(0.0 < s < 1.0 for s in res["share"])

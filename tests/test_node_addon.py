@@ -38,7 +38,7 @@ console.log(JSON.stringify({ keys: Object.keys(addon), version: addon.version(),
               "readTemps", "readFlags", "readPopulations", "guardCount", "sync", "writeFields", "sampleFlow",
               "particlesAlloc", "particlesStep", "readParticles", "writeParticles", "streamlines", "batchCreate",
               "batchInstances", "batchDestroy", "batchStep", "batchRunGraph", "batchAutotune", "batchSetLaunchShape",
-              "bakeSdf"):
+              "bakeSdf", "vlmLoads"):
         assert k in out["keys"]
     assert out["version"] == aerolbm.load().aero_version()
     assert "aero_create failed" in out["msg"] and "tau0" in out["msg"]
@@ -93,3 +93,43 @@ console.log(JSON.stringify({{view: Buffer.from(view.buffer).toString('hex'), psi
     st, em = FL.stream_uniform(p, FL.FlowMode.Vortices if mode == 4 else FL.FlowMode.Streamlines, 0, 0.5)
     assert bytes.fromhex(js["stream"]) == bytes(st)
     assert bytes.fromhex(js["em"]) == em.tobytes()
+
+
+W17_SETUPS = [
+    dict(frontDeg=0, rearFlapDeg=0, rearMainDeg=0, rideH=0.12, rearRideH=0.14, gurney=False, frontOn=True,
+         rearOn=True, rearTwistDeg=0, flapScale=1),
+    dict(frontDeg=8.5, rearFlapDeg=-15, rearMainDeg=-3.5, rideH=0.08, rearRideH=0.26, gurney=True, frontOn=True,
+         rearOn=True, rearTwistDeg=-16, flapScale=0.72),
+    dict(frontDeg=-9, rearFlapDeg=9, rearMainDeg=3, rideH=0.2, rearRideH=0.1, gurney=False, frontOn=True,
+         rearOn=True, rearTwistDeg=6, flapScale=1),
+]
+
+
+def test_w17_lattices_match_python_host(tmp_path):
+    """The JavaScript lattice builder (V8's Math functions, as the reference runs) and
+    the Python one (the C library's cos / sin / atan) give the same horseshoes: bit for
+    bit but for rare last-bit differences of those functions."""
+    import numpy as np
+
+    from aerolbm import vlm as V
+
+    rig_path = ROOT / "tests" / "golden" / "w17_rig.json"
+    out = tmp_path / "lat.f64"
+    node(f"""
+const fs = require('fs');
+const {{ w17Lattices }} = require('./hip-engine');
+const rig = JSON.parse(fs.readFileSync({json.dumps(str(rig_path))}));
+const parts = [];
+for (const s of {json.dumps(W17_SETUPS)}) {{ const l = w17Lattices(rig, s); parts.push(l.front, l.rear); }}
+const o = new Float64Array(parts.reduce((a, p) => a + p.length, 0));
+let k = 0;
+for (const p of parts) {{ o.set(p, k); k += p.length; }}
+fs.writeFileSync({json.dumps(str(out))}, Buffer.from(o.buffer));
+""")
+    got = np.fromfile(out, np.float64)
+    rig = json.loads(rig_path.read_text())
+    ref = np.concatenate([np.concatenate(V.w17_lattices(rig, V.W17WingSetup(**s))).ravel() for s in W17_SETUPS])
+    assert got.shape == ref.shape == (((240 + 240) * 3 + 80) * 20,)
+    differ = got.view(np.uint64) != ref.view(np.uint64)
+    assert differ.sum() <= 0.001 * got.size
+    assert np.abs(got - ref).max() <= 4 * np.finfo(np.float64).eps * max(1.0, np.abs(ref).max())
