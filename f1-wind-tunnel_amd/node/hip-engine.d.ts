@@ -103,8 +103,9 @@ export declare function aeroMap(opts: {
   prims?: (front: number, rear: number) => Uint8Array | null; settle: number; measure: number;
   front?: number[]; rear?: number[]; storage?: number; stepsPerGraph?: number; launchShape?: number;
   uLb?: number; tau0?: number; smag?: number; kmh?: number;
-  /** W17 wings: each cell's bal is frontAxleShare of the panel-method loads at its ride heights. */
-  wings?: { rig: RigData; setup: W17WingSetup; speed: number };
+  /** W17 wings: each cell is the point's forces through withPanelWings with the panel-method
+   *  loads at its ride heights, and bal their frontAxleShare (main.ts:669-675). */
+  wings?: { rig: RigData; setup: W17WingSetup; sidepods?: number; cooling?: number };
 }): { cells: AeroMapCell[]; deviceMs: number };
 
 /** bakeGrid of scripts/bake-car.ts:237-316 on the GPU (aero_bake_sdf): f16 words, x fastest. */
@@ -150,3 +151,21 @@ export declare function w17WingLoadsMany(rig: RigData, setups: W17WingSetup[], s
   { front: WingLoads; rear: WingLoads }[];
 /** frontAxleShare (main.ts:618-625). */
 export declare function frontAxleShare(loads: { front: WingLoads; rear: WingLoads }): number | undefined;
+
+// ---------- Force post-processing (src/main.ts:586-689) ----------
+/** withPanelWings with its body-coefficient state. */
+export declare class PanelWings {
+  cd: number; cl: number; n: number;
+  reset(): void;
+  apply(f: Forces, wingLoads: { front: WingLoads; rear: WingLoads } | null, kmh: number, sidepods?: number,
+        cooling?: number, w17?: boolean): Forces;
+}
+export interface TelemetrySample { t: number; drag: number; down: number; fd: number; fdd: number; rd: number;
+                                   rdd: number; bd: number; bdd: number }
+/** engine.onForces without the drawing. */
+export declare class ForceTelemetry {
+  constructor(nowMs?: number);
+  recent: TelemetrySample[]; history: TelemetrySample[]; smoothed: TelemetrySample; warmupUntil: number;
+  reset(nowMs: number, panel?: PanelWings): void;
+  onForces(nowMs: number, lbm: Forces, post?: (f: Forces) => Forces): TelemetrySample | null;
+}

@@ -418,21 +418,27 @@ function aeroMap(opts) {
     let ms = batch.runGraph(opts.settle, opts.stepsPerGraph);
     for (const e of batch.engines) e.readForces(true);
     ms += batch.runGraph(opts.measure, opts.stepsPerGraph);
-    // bal: the front-axle share of the panel-method wing load at each pair when the W17
-    // wings are given (opts.wings = { rig, setup, speed }), else 0.5 (aeromap.ts:101)
-    const bal = pairs.map(() => 0.5);
+    // With the W17 wings given (opts.wings = { rig, setup, sidepods?, cooling? }) a cell is
+    // what the reference's map samples there (main.ts:669-675): the point's forces through
+    // withPanelWings with the panel-method loads at its ride heights, and bal their
+    // front-axle share; otherwise the tunnel's forces and bal 0.5 (aeromap.ts:101).
+    const kmh = opts.kmh !== undefined ? opts.kmh : 250;
+    let loads = null;
     if (opts.wings) {
       const w = opts.wings;
       const setups = pairs.map(([f, r]) => Object.assign({}, w.setup, { rideH: f, rearRideH: r }));
-      w17WingLoadsMany(w.rig, setups, setups.map(() => w.speed)).forEach((ld, i) => {
-        const share = frontAxleShare(ld);
-        if (share !== undefined) bal[i] = share;
-      });
+      loads = w17WingLoadsMany(w.rig, setups, setups.map(() => kmh / 3.6));
     }
     const cells = pairs.map(([f, r], i) => {
-      const F = batch.engines[i].readForces(true);
-      return { front: f, rear: r, down: F.downforceN / KGF, eff: F.downforceN / Math.max(F.dragN, 1), bal: bal[i],
-               forces: F };
+      let F = batch.engines[i].readForces(true);
+      let bal = 0.5;
+      if (loads) {
+        F = new PanelWings().apply(F, loads[i], kmh, opts.wings.sidepods || 0,
+                                   opts.wings.cooling !== undefined ? opts.wings.cooling : 1);
+        const share = frontAxleShare(loads[i]);
+        if (share !== undefined) bal = share;
+      }
+      return { front: f, rear: r, down: F.downforceN / KGF, eff: F.downforceN / Math.max(F.dragN, 1), bal, forces: F };
     });
     return { cells, deviceMs: ms };
   } finally {
@@ -639,6 +645,74 @@ function frontAxleShare(loads) {
   return wings.reduce((a, w) => a + w.downforceN * (VLM_RIG.rearAxleX - w.copX) / wb, 0) / total;
 }
 
+// ---------- Force post-processing (src/main.ts:586-689) ----------
+
+// main.ts:20-28: ΔCdA / ΔClA per cooling and sidepod package, W17 body calibration
+const COOLING_COEF = [{ cd: -0.03, cl: 0.01 }, { cd: 0, cl: 0 }, { cd: 0.055, cl: -0.02 }];
+const SIDEPOD_COEF = [{ cd: 0, cl: 0 }, { cd: -0.05, cl: 0.035 }];
+const BODY_CDA_SCALE = 0.69;
+const BODY_CLA_FLOOR = 0.31;
+const AVG_WINDOW_MS = 3000, CHART_POINTS = 220, WARMUP_MS = 2500; // main.ts:634-645
+const FORCE_KEYS = ["drag", "down", "fd", "fdd", "rd", "rdd", "bd", "bdd"];
+
+/** withPanelWings (main.ts:586-616) with its body-coefficient state. */
+class PanelWings {
+  constructor() { this.cd = 0; this.cl = 0; this.n = 0; }
+  reset() { this.n = 0; }
+  /** The forces the app shows for `f` (engine Forces) given the wing loads ({front, rear}
+   *  or null), the speed and the sidepod / cooling package; unchanged off the W17. */
+  apply(f, wingLoads, kmh, sidepods = 0, cooling = 1, w17 = true) {
+    if (!w17 || !wingLoads) return f;
+    const q = 0.5 * AIR_DENSITY * (kmh / 3.6) ** 2;
+    const a = this.n < 20 ? 0.15 : this.n < 60 ? 0.04 : 0.011;
+    this.cd += (f.body.drag / q - this.cd) * (this.n === 0 ? 1 : a);
+    this.cl += (f.body.down / q - this.cl) * (this.n === 0 ? 1 : a);
+    this.n++;
+    const pods = SIDEPOD_COEF[sidepods] || SIDEPOD_COEF[0];
+    const cool = COOLING_COEF[cooling] || COOLING_COEF[1];
+    const fw = { drag: wingLoads.front.dragN, down: wingLoads.front.downforceN };
+    const rw = { drag: wingLoads.rear.dragN, down: wingLoads.rear.downforceN };
+    const body = { drag: (this.cd * BODY_CDA_SCALE + pods.cd + cool.cd) * q,
+                   down: (this.cl + BODY_CLA_FLOOR + pods.cl + cool.cl) * q };
+    return Object.assign({}, f, { body, frontWing: fw, rearWing: rw, dragN: body.drag + fw.drag + rw.drag,
+                                  downforceN: body.down + fw.down + rw.down });
+  }
+}
+
+/** engine.onForces (main.ts:669-689) without the drawing: warm-up gate, 3 s window
+ *  mean (`smoothed`), chart points moved 35 % towards each reading (`history`). */
+class ForceTelemetry {
+  constructor(nowMs = 0) {
+    this.recent = []; this.history = []; this.warmupUntil = nowMs + WARMUP_MS;
+    this.smoothed = { t: 0 };
+    for (const k of FORCE_KEYS) this.smoothed[k] = 0;
+  }
+  /** resetChart (main.ts:647-651): the chart and, when given, the body coefficients. */
+  reset(nowMs, panel) {
+    this.history.length = 0; this.recent.length = 0;
+    if (panel) panel.reset();
+    this.warmupUntil = nowMs + WARMUP_MS;
+  }
+  /** One reading; `post` (withPanelWings) runs after the warm-up gate. Returns the raw
+   *  sample, or null while warming up. */
+  onForces(nowMs, lbm, post) {
+    if (nowMs < this.warmupUntil) return null;
+    const f = post ? post(lbm) : lbm;
+    const raw = { t: nowMs, drag: f.dragN, down: f.downforceN, fd: f.frontWing.drag, fdd: f.frontWing.down,
+                  rd: f.rearWing.drag, rdd: f.rearWing.down, bd: f.body.drag, bdd: f.body.down };
+    this.recent.push(raw);
+    while (this.recent.length > 1 && nowMs - this.recent[0].t > AVG_WINDOW_MS) this.recent.shift();
+    for (const k of FORCE_KEYS) this.smoothed[k] = this.recent.reduce((acc, r) => acc + r[k], 0) / this.recent.length;
+    const last = this.history[this.history.length - 1];
+    const point = Object.assign({}, raw);
+    if (last) for (const k of FORCE_KEYS) point[k] = last[k] + (raw[k] - last[k]) * 0.35;
+    this.history.push(point);
+    if (this.history.length > CHART_POINTS) this.history.shift();
+    return raw;
+  }
+}
+
 module.exports = { HipAeroEngine, HipAeroBatch, aeroMap, bakeGrid, buildLattice, solveLattices, wingLoads, withGurney,
-                   w17Poses, w17Lattices, w17WingLoads, w17WingLoadsMany, frontAxleShare, FRONT_SPAN, REAR_SPAN, NO_LOAD, AEROMAP_FRONT, AEROMAP_REAR, ShapeKind, Storage, FlowMode, ColorField, packShape, packPSim, packStream, viewUniform,
+                   w17Poses, w17Lattices, w17WingLoads, w17WingLoadsMany, frontAxleShare, FRONT_SPAN, REAR_SPAN, NO_LOAD,
+                   PanelWings, ForceTelemetry, AEROMAP_FRONT, AEROMAP_REAR, ShapeKind, Storage, FlowMode, ColorField, packShape, packPSim, packStream, viewUniform,
                    addon, QUALITIES, PARTICLES, MAX_LINES, STREAM_STEPS, LATTICE_U, TAU0, SMAG };

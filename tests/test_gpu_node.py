@@ -166,3 +166,44 @@ fs.writeFileSync({json.dumps(str(op))}, JSON.stringify({{ sizes: lats.map((p) =>
         wing = res["many"][k // 2]["front" if k % 2 == 0 else "rear"]
         assert [wing[n] for n in keys] == got
     assert all(0.0 < s < 1.0 for s in res["share"])
+
+
+def test_node_w17_map_post_processing_matches_python():
+    """PanelWings + w17WingLoadsMany in the Node host against aerolbm.forces.w17_map_cells
+    on the same per-point forces."""
+    from aerolbm import aeromap as AM
+    from aerolbm import forces as FP
+    from aerolbm import vlm as V
+    from aerolbm.engine import ForcePair, Forces
+
+    rig_path = ROOT / "tests" / "golden" / "w17_rig.json"
+    pairs = [(f, r) for f in AM.REFERENCE_FRONT for r in AM.REFERENCE_REAR]
+    bodies = [(800.0 + 10 * k, -200.0 + 5 * k) for k in range(25)]
+    script = f"""
+const fs = require('fs');
+const {{ PanelWings, w17WingLoadsMany, frontAxleShare }} = require('./hip-engine');
+const rig = JSON.parse(fs.readFileSync({json.dumps(str(rig_path))}));
+const pairs = {json.dumps(pairs)}, bodies = {json.dumps(bodies)};
+const base = {{ frontDeg: 0, rearFlapDeg: 0, rearMainDeg: 0, rideH: 0.12, rearRideH: 0.14, gurney: false,
+               frontOn: true, rearOn: true, rearTwistDeg: 0, flapScale: 1 }};
+const setups = pairs.map(([f, r]) => Object.assign({{}}, base, {{ rideH: f, rearRideH: r }}));
+const loads = w17WingLoadsMany(rig, setups, setups.map(() => 250 / 3.6));
+const cells = pairs.map((_, i) => {{
+  const F = {{ dragN: 0, downforceN: 0, body: {{ drag: bodies[i][0], down: bodies[i][1] }},
+              frontWing: {{ drag: 0, down: 0 }}, rearWing: {{ drag: 0, down: 0 }} }};
+  const s = new PanelWings().apply(F, loads[i], 250, 1, 0);
+  return {{ down: s.downforceN / 9.81, eff: s.downforceN / Math.max(s.dragN, 1), bal: frontAxleShare(loads[i]) }};
+}});
+console.log(JSON.stringify(cells));
+"""
+    r = subprocess.run(["node", "-e", script], cwd=NODE_DIR, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    js = json.loads(r.stdout)
+    lbm = [Forces(dragN=0.0, downforceN=0.0, body=ForcePair(*b), frontWing=ForcePair(0.0, 0.0),
+                  rearWing=ForcePair(0.0, 0.0)) for b in bodies]
+    rig = json.loads(rig_path.read_text())
+    py = FP.w17_map_cells(lbm, pairs, rig, V.W17WingSetup(), 250.0, sidepods=1, cooling=0)
+    for a, b in zip(js, py):
+        assert a["down"] == pytest.approx(b["down"], rel=1e-12)
+        assert a["eff"] == pytest.approx(b["eff"], rel=1e-12)
+        assert a["bal"] == pytest.approx(b["bal"], rel=1e-12)

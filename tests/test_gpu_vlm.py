@@ -156,3 +156,28 @@ def test_ride_height_map_batch():
     assert (np.diff(front[:, 0]) < 0).all()
     bal = AM.wing_balance(RIG, V.W17WingSetup(), pairs, 250 / 3.6)
     assert len(bal) == 25 and all(0.0 < b < 1.0 for b in bal)
+
+
+def test_w17_map_cells_post_processing():
+    """w17_map_cells: each point's forces through withPanelWings with the panel loads at
+    its ride heights (one batched solve), bal their front-axle share (main.ts:669-675,
+    aeromap.ts:96-101)."""
+    from aerolbm import aeromap as AM
+    from aerolbm import forces as FP
+    from aerolbm.engine import ForcePair, Forces
+
+    pairs = [(f, r) for f in AM.REFERENCE_FRONT for r in AM.REFERENCE_REAR]
+    lbm = [Forces(dragN=0.0, downforceN=0.0, body=ForcePair(800.0 + 10 * k, -200.0 + 5 * k),
+                  frontWing=ForcePair(0.0, 0.0), rearWing=ForcePair(0.0, 0.0)) for k in range(25)]
+    kmh = 250.0
+    cells = FP.w17_map_cells(lbm, pairs, RIG, V.W17WingSetup(), kmh, sidepods=1, cooling=0)
+    q = 0.5 * 1.225 * (kmh / 3.6) ** 2
+    for k, (c, (f, r)) in enumerate(zip(cells, pairs)):
+        ld = V.w17_wing_loads(RIG, V.W17WingSetup(rideH=f, rearRideH=r), kmh / 3.6)
+        body_down = (lbm[k].body.down / q + 0.31 + 0.035 + 0.01) * q
+        body_drag = (lbm[k].body.drag / q * 0.69 - 0.05 - 0.03) * q
+        down = body_down + ld["front"].downforceN + ld["rear"].downforceN
+        drag = body_drag + ld["front"].dragN + ld["rear"].dragN
+        assert c["down"] == pytest.approx(down / 9.81, rel=1e-12)
+        assert c["eff"] == pytest.approx(down / max(drag, 1.0), rel=1e-12)
+        assert c["bal"] == V.front_axle_share(ld)
