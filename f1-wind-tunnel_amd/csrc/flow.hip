@@ -12,6 +12,7 @@
 // v / length(v), so the results are bit-identical to the f32 CPU restatement
 // (the sines and cosines of emitter-ring seeds aside).
 #include "aero_internal.h"
+#include "sdf.h"
 
 namespace aero {
 
@@ -230,9 +231,11 @@ __device__ void seed_of(const aero_stream& st, const float4* em, uint32_t line, 
         const float4 e = em[ei];
         const float ang = (((float)k + 0.5f) / (float)st.per_emitter) * 6.2831853f + st.time * 0.6f;
         const float ring = 0.35f + (0.65f * (float)(k % 3u)) / 2.0f;
+        float sa, ca;  // FP64-evaluated, as the oracle evaluates the ring angle
+        sdf::sincos_r64(ang, &sa, &ca);
         x = e.x + (0.0f * e.w) * ring;
-        y = e.y + (__builtin_cosf(ang) * e.w) * ring;
-        z = e.z + (__builtin_sinf(ang) * e.w) * ring;
+        y = e.y + (ca * e.w) * ring;
+        z = e.z + (sa * e.w) * ring;
         return;
     }
     const uint32_t row = line / st.cols, col = line % st.cols;

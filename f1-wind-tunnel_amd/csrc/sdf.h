@@ -29,6 +29,37 @@ __host__ __device__ __forceinline__ float smoothstep(float e0, float e1, float x
     return t * t * (3.0f - 2.0f * t);
 }
 
+// sin and cos of an f32 angle, evaluated in FP64 and rounded once to f32, so that this
+// device code and the CPU oracle give the same bits where the device's cosf/sinf and a C
+// library's may differ in the last one.
+// Cody-Waite reduction by pi/2 with fdlibm's two-part constant (the first part has 33
+// significant bits, so k * P1 is exact for |k| < 2^20), then the Taylor series of sin
+// to r^19 and of cos to r^20 on |r| <= pi/4 (truncation below 2^-70), Horner order.
+__host__ __device__ __forceinline__ void sincos_r64(float a, float* sn, float* cs) {
+    const double x = (double)a;
+    if (!(__builtin_fabs(x) <= 1e9)) {
+        *sn = *cs = __builtin_nanf("");
+        return;
+    }
+    const double kd = __builtin_rint(x * 0x1.45f306dc9c883p-1);  // x * 2/pi
+    const double r = (x - kd * 0x1.921fb544p+0) - kd * 0x1.0b4611a626331p-34;
+    const double r2 = r * r;
+    const double ps = r2 * (-0x1.5555555555555p-3 + r2 * (0x1.1111111111111p-7 + r2 * (-0x1.a01a01a01a01ap-13 +
+                      r2 * (0x1.71de3a556c734p-19 + r2 * (-0x1.ae64567f544e4p-26 + r2 * (0x1.6124613a86d09p-33 +
+                      r2 * (-0x1.ae7f3e733b81fp-41 + r2 * (0x1.952c77030ad4ap-49 + r2 * -0x1.2f49b46814157p-57))))))));
+    const double s = r + r * ps;
+    const double pc = r2 * (-0x1.0000000000000p-1 + r2 * (0x1.5555555555555p-5 + r2 * (-0x1.6c16c16c16c17p-10 +
+                      r2 * (0x1.a01a01a01a01ap-16 + r2 * (-0x1.27e4fb7789f5cp-22 + r2 * (0x1.1eed8eff8d898p-29 +
+                      r2 * (-0x1.93974a8c07c9dp-37 + r2 * (0x1.ae7f3e733b81fp-45 + r2 * (-0x1.6827863b97d97p-53 +
+                      r2 * 0x1.e542ba4020225p-62)))))))));
+    const double c = 1.0 + pc;
+    const int q = (int)((long long)kd & 3);
+    const double so = q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+    const double co = q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+    *sn = (float)so;
+    *cs = (float)co;
+}
+
 __host__ __device__ __forceinline__ float sdBox(v3 p, v3 b) {
     const v3 q = mk3(fabs_(p.x) - b.x, fabs_(p.y) - b.y, fabs_(p.z) - b.z);
     return length(mk3(fmax_(q.x, 0.0f), fmax_(q.y, 0.0f), fmax_(q.z, 0.0f))) + fmin_(fmax_(q.x, fmax_(q.y, q.z)), 0.0f);

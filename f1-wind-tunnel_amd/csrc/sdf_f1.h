@@ -2,8 +2,8 @@
 // Reference definitions: src/engine/f1.ts:20-320 (sdF1 and its parts) and
 // src/engine/f1.ts:397-407 (f1Emission).  f32 arithmetic in the reference's order.
 // The rake angle and the rear-wing main-plane angle depend on the shape only; their
-// cos/sin arrive precomputed in F1Ctx.  The per-point angles of sdWingZ use the
-// device cosf/sinf.
+// cos/sin arrive precomputed in F1Ctx.  The per-point angles of sdWingZ use
+// sincos_r64 (sdf.h), which the CPU oracle evaluates to the same bits.
 #pragma once
 #include "sdf.h"
 
@@ -50,8 +50,9 @@ __device__ __forceinline__ float sdWingZ(v3 p, v2 le, float chord, float ang, fl
                                          float rise, float taper, float curl) {
     const float s = smoothstep(curl, z1, p.z);
     const float a = ang + rise * s * 0.6f;
-    return sdWingCS(p, mk2(le.x, le.y + rise * s), chord * (1.0f - taper * s), cosf(a), sinf(a), thick, z0, z1) *
-           0.85f;
+    float sa, ca;
+    sincos_r64(a, &sa, &ca);
+    return sdWingCS(p, mk2(le.x, le.y + rise * s), chord * (1.0f - taper * s), ca, sa, thick, z0, z1) * 0.85f;
 }
 __device__ __forceinline__ float tunnelRoof(float x, float b) {
     if (x < 2.5f) return mix(b + 0.12f, b + 0.05f, smoothstep(1.55f, 2.5f, x));

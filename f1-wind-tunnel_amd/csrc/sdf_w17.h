@@ -90,7 +90,8 @@ __device__ __forceinline__ v3 rig_rotate_inv(const aero_rig& r, v3 p, uint32_t p
     const float a = -rig_angle(r, part, p.z);
     const float k = rig_scale(r, part);
     const v2 h = rig_hinge(r, part, p.z);
-    const float c = cosf(a), s = sinf(a);
+    float s, c;
+    sincos_r64(a, &s, &c);
     v2 d = mk2(p.x - h.x, p.y - h.y);
     d = mk2(c * d.x - s * d.y, s * d.x + c * d.y);
     d = mk2(d.x / k, d.y / k);

@@ -28,6 +28,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include "sincos_r64.h"
+
 typedef struct { uint32_t nx, ny, nz, n; float ox, oy, oz, h; } fo_grid;
 typedef struct { float uIn, vReal, cpMin, cpMax, tMin, tMax, speedMax, vortMax; } fo_view;
 typedef struct {
@@ -254,9 +256,11 @@ static void seed_of(const fo_stream* st, const float* em, uint32_t line, float* 
         const float* e = em + 4 * ei;
         const float a = (((float)k + 0.5f) / (float)st->perEmitter) * 6.2831853f + st->time * 0.6f;
         const float ring = 0.35f + (0.65f * (float)(k % 3u)) / 2.0f;
+        float sa, ca; /* sincos_r64, as the device evaluates the ring angle */
+        sincos_r64(a, &sa, &ca);
         p[0] = e[0] + (0.0f * e[3]) * ring;
-        p[1] = e[1] + (cosf(a) * e[3]) * ring;
-        p[2] = e[2] + (sinf(a) * e[3]) * ring;
+        p[1] = e[1] + (ca * e[3]) * ring;
+        p[2] = e[2] + (sa * e[3]) * ring;
         return;
     }
     const uint32_t row = line / st->cols, col = line % st->cols;

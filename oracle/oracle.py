@@ -153,10 +153,17 @@ class SDFOracle:
         L.orc_half_to_float.restype = C.c_float
         L.orc_half_to_float.argtypes = [C.c_uint16]
         self._car = None
+        L.so_sincos.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         for name, nargs in (("orc_sd_wing", 10), ("orc_sd_wheel", 8), ("orc_sd_roundbox", 7)):
             fn = getattr(L, name)
             fn.restype = C.c_float
             fn.argtypes = [C.c_float] * nargs
+
+    def sincos(self, a: float):
+        """(sin, cos) of the f32 angle a as the voxelizer evaluates per-point angles."""
+        sn, cs = C.c_float(), C.c_float()
+        self.lib.so_sincos(C.c_float(a), C.byref(sn), C.byref(cs))
+        return sn.value, cs.value
 
     def set_car(self, body: np.ndarray, atlas: np.ndarray, atlas_dims, rig) -> None:
         """W17 grids (u16 f16 words) and the Rig struct (any ctypes struct with the

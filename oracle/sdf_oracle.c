@@ -32,6 +32,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include "sincos_r64.h"
+
 #define EXPORT __attribute__((visibility("default")))
 
 typedef struct { uint32_t nx, ny, nz, n; float ox, oy, oz, h; } og_grid;
@@ -157,6 +159,9 @@ static float sdTaperC(f3 p, float x0, float x1, f2 c0, f2 c1, f2 s0, f2 s1, floa
 static const og_shape* g_sh; /* the `shape` uniform of the current voxelize call */
 
 static f3 add3(f3 a, f3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+/* The per-point sin/cos (sincos_r64.h), exported for its test. */
+EXPORT void so_sincos(float a, float* sn, float* cs) { sincos_r64(a, sn, cs); }
+
 static float smoothstepf(float e0, float e1, float x) {
     float t = clampf_((x - e0) / (e1 - e0), 0.0f, 1.0f);
     return t * t * (3.0f - 2.0f * t);
@@ -187,8 +192,9 @@ static float sdEllipsoid(f3 p, f3 r) {
 static float sdWingZ(f3 p, f2 le, float chord, float ang, float thick, float z0, float z1, float rise, float taper,
                      float curl) {
     float s = smoothstepf(curl, z1, p.z);
-    return sdWing(p, v2(le.x, le.y + rise * s), chord * (1.0f - taper * s), ang + rise * s * 0.6f, thick, z0, z1) *
-           0.85f;
+    float sa, ca; /* per-point angle: sincos_r64, as the device evaluates it */
+    sincos_r64(ang + rise * s * 0.6f, &sa, &ca);
+    return sdWingCS(p, v2(le.x, le.y + rise * s), chord * (1.0f - taper * s), ca, sa, thick, z0, z1) * 0.85f;
 }
 /* f1.ts:70-74 */
 static float tunnelRoof(float x, float b) {
@@ -625,7 +631,8 @@ static f3 rigRotateInv(f3 p, uint32_t part) { /* rigRotate(p, part, -1) */
     float a = -rigAngle(part, p.z);
     float k = rigScale(part);
     f2 h = rigHinge(part, p.z);
-    float c = cosf(a), s = sinf(a);
+    float s, c;
+    sincos_r64(a, &s, &c);
     f2 d = sub2(v2(p.x, p.y), h);
     d = v2(c * d.x - s * d.y, s * d.x + c * d.y);
     d = v2(d.x / k, d.y / k);

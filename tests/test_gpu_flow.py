@@ -1,8 +1,7 @@
 """GPU parity of the field consumers (csrc/flow.hip, through the C ABI) with the
 f32 oracle (oracle/flow_oracle.c) on the same fields and temperatures: probes,
-particle dispatches and rake streamlines bit-identical; emitter-ring streamlines
-(device cosf / sinf) within a stated tolerance; closed-form motion in a uniform
-flow at the F1 grid size; argument errors."""
+particle dispatches, rake and emitter-ring streamlines bit-identical; closed-form
+motion in a uniform flow at the F1 grid size; argument errors."""
 import numpy as np
 import pytest
 
@@ -117,9 +116,9 @@ def test_rake_streamlines_match_oracle_bit_for_bit():
     e.close()
 
 
-def test_emitter_streamlines_match_oracle_within_tolerance():
-    """Seeds on emitter rings use device cosf / sinf (within 2 ulp of the host libm),
-    so the seeds agree to 1e-6 m and the first 40 points of every line to 1e-4 m."""
+def test_emitter_streamlines_match_oracle():
+    """Seeds on emitter rings (the ring angle's sin / cos evaluated in FP64 on both
+    sides): the lines are bit-identical to the oracle's."""
     print("streamlines emitters", flush=True)
     e = scene_engine()
     f, t = e.fields(), e.temps()
@@ -130,9 +129,8 @@ def test_emitter_streamlines_match_oracle_within_tolerance():
     st.per_emitter, st.emitters, st.step_len, st.time, st.rows, st.cols = 53, 3, 0.022, 1.7, 1, 1
     got = e.streamlines(st, v, em)
     want = FO.streamlines(ogrid(e), f, t, st, v, em)
-    np.testing.assert_allclose(got[:, 0, :3], want[:, 0, :3], rtol=0, atol=1e-6)
-    both = (got[:, :40, 3] >= 0) & (want[:, :40, 3] >= 0)
-    np.testing.assert_allclose(got[:, :40, :3][both], want[:, :40, :3][both], rtol=0, atol=1e-4)
+    assert np.array_equal(got, want, equal_nan=True)
+    assert (got[:, :, 3] >= 0).sum() > 159 * 20  # lines do travel
     e.close()
 
 

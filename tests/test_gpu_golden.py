@@ -1,7 +1,6 @@
 """The library against the committed golden fixtures (tests/golden): exact-mode steps
 reproduce the stored populations, fields, fixed-point and FP64 force sums bit for bit;
-fast mode within 1e-5 relative; heat bit for bit; the voxelizer bit for bit (kind 0
-outside the stated band of its device sin/cos)."""
+fast mode within 1e-5 relative; heat bit for bit; the voxelizer bit for bit."""
 import math
 from pathlib import Path
 
@@ -77,11 +76,6 @@ def test_voxel_fixture(name, kind):
     e.set_shape(s)
     solid, emit = e.flags()
     want = d[f"{name}_solid"].astype(np.uint32)
-    diff = solid != want
-    if kind == 0:  # device sin/cos of the per-point wing angles: cells within 2e-5 m of 0.3h may differ
-        band = np.abs(d[f"{name}_sdf"] - np.float32(0.3) * d[f"{name}_h"]) < 2e-5
-        assert band[diff].all()
-    else:
-        assert not diff.any()
-    assert np.array_equal(emit[~diff], d[f"{name}_emit"][~diff])
+    assert np.array_equal(solid, want)
+    assert np.array_equal(emit, d[f"{name}_emit"])
     e.close()
