@@ -32,53 +32,82 @@ __device__ __forceinline__ float heat_at(const HeatArgs& h, int x, int y, int z)
     return x < 0 ? 0.0f : h.tin[heat_idx(h, x, y, z)];
 }
 
+// Each thread handles the cells (x, y, z) and (x, y, z + 1): the loads of both cells go
+// out before either waits, so every wavefront keeps twice the memory requests in flight
+// through the two dependent rounds (velocity, then the back-traced gather).  The
+// arithmetic of each cell is unchanged.
+constexpr int kHeatCells = 2;
+
 __global__ void __launch_bounds__(256) k_heat(HeatArgs h) {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int z = blockIdx.z;
+    const int z0 = blockIdx.z * kHeatCells;
     if (x >= (int)h.nx || y >= (int)h.ny) return;
-    const uint32_t idx = (uint32_t)x + h.nx * ((uint32_t)y + h.ny * (uint32_t)z);
-    // Loads that do not depend on the velocity go out first, together with it: the
-    // six neighbour temperatures and the flag byte.
     const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
-    float nbv[6];
+    bool live[kHeatCells];
+    uint32_t idx[kHeatCells], fl[kHeatCells];
+    float nbv[kHeatCells][6];
+    float4 fv[kHeatCells];
+    // Loads that do not depend on the velocity go out first, together with it: the six
+    // neighbour temperatures and the flag byte, for both cells.
 #pragma unroll
-    for (int k = 0; k < 6; k++) nbv[k] = heat_at(h, x + off[k][0], y + off[k][1], z + off[k][2]);
-    const uint32_t fl = h.flags[idx];
-    const float4 fv = h.fields[idx];
-    if (fv.w <= 0.0f) { __builtin_nontemporal_store(h.emit[idx], h.tout + idx); return; }
-    const float gx = (float)x - fv.x * h.dt, gy = (float)y - fv.y * h.dt, gz = (float)z - fv.z * h.dt;
-    const float flx = __builtin_floorf(gx), fly = __builtin_floorf(gy), flz = __builtin_floorf(gz);
-    const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
-    const float fx = gx - (float)ix, fy = gy - (float)iy, fz = gz - (float)iz;
-    float tv[8];
+    for (int c = 0; c < kHeatCells; c++) {
+        const int z = z0 + c;
+        live[c] = z < (int)h.nz;
+        const int zs = live[c] ? z : z0;  // a dead second cell repeats the first one's loads
+        idx[c] = (uint32_t)x + h.nx * ((uint32_t)y + h.ny * (uint32_t)zs);
 #pragma unroll
-    for (int k = 0; k < 8; k++) tv[k] = heat_at(h, ix + (k & 1), iy + ((k >> 1) & 1), iz + ((k >> 2) & 1));
-    float t = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
-        const float w = (ox ? fx : 1.0f - fx) * (oy ? fy : 1.0f - fy) * (oz ? fz : 1.0f - fz);
-        t = t + tv[k] * w;
+        for (int k = 0; k < 6; k++) nbv[c][k] = heat_at(h, x + off[k][0], y + off[k][1], zs + off[k][2]);
+        fl[c] = h.flags[idx[c]];
+        fv[c] = h.fields[idx[c]];
     }
-    float nb = 0.0f;
+    float fx[kHeatCells], fy[kHeatCells], fz[kHeatCells], tv[kHeatCells][8];
 #pragma unroll
-    for (int k = 0; k < 6; k++) nb = nb + nbv[k];
-    float src = 0.0f;
-    if (fl & FL_HOTNB) {
+    for (int c = 0; c < kHeatCells; c++) {
+        const int zs = live[c] ? z0 + c : z0;
+        const float gx = (float)x - fv[c].x * h.dt, gy = (float)y - fv[c].y * h.dt, gz = (float)zs - fv[c].z * h.dt;
+        const float flx = __builtin_floorf(gx), fly = __builtin_floorf(gy), flz = __builtin_floorf(gz);
+        const int ix = (int)flx, iy = (int)fly, iz = (int)flz;
+        fx[c] = gx - (float)ix;
+        fy[c] = gy - (float)iy;
+        fz[c] = gz - (float)iz;
 #pragma unroll
-        for (int k = 0; k < 6; k++) {
-            const int cx = x + off[k][0], cy = y + off[k][1], cz = z + off[k][2];
-            if (cx >= 0 && cy >= 0 && cx < (int)h.nx && cy < (int)h.ny && cz < (int)h.nz) {
-                const float e = h.emit[heat_idx(h, cx, cy, cz)];
-                src = src > e ? src : e;
+        for (int k = 0; k < 8; k++) tv[c][k] = heat_at(h, ix + (k & 1), iy + ((k >> 1) & 1), iz + ((k >> 2) & 1));
+    }
+#pragma unroll
+    for (int c = 0; c < kHeatCells; c++) {
+        if (!live[c]) continue;
+        const int z = z0 + c;
+        if (fv[c].w <= 0.0f) {
+            __builtin_nontemporal_store(h.emit[idx[c]], h.tout + idx[c]);
+            continue;
+        }
+        float t = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int ox = k & 1, oy = (k >> 1) & 1, oz = (k >> 2) & 1;
+            const float w = (ox ? fx[c] : 1.0f - fx[c]) * (oy ? fy[c] : 1.0f - fy[c]) * (oz ? fz[c] : 1.0f - fz[c]);
+            t = t + tv[c][k] * w;
+        }
+        float nb = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 6; k++) nb = nb + nbv[c][k];
+        float src = 0.0f;
+        if (fl[c] & FL_HOTNB) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) {
+                const int cx = x + off[k][0], cy = y + off[k][1], cz = z + off[k][2];
+                if (cx >= 0 && cy >= 0 && cx < (int)h.nx && cy < (int)h.ny && cz < (int)h.nz) {
+                    const float e = h.emit[heat_idx(h, cx, cy, cz)];
+                    src = src > e ? src : e;
+                }
             }
         }
+        t = t * (1.0f - h.diffuse) + (nb / 6.0f) * h.diffuse;
+        if (src > 0.0f) t = t * (1.0f - 0.35f) + src * 0.35f;
+        const float o = t * (1.0f - h.cool);
+        __builtin_nontemporal_store(o > 0.0f ? o : 0.0f, h.tout + idx[c]);
     }
-    t = t * (1.0f - h.diffuse) + (nb / 6.0f) * h.diffuse;
-    if (src > 0.0f) t = t * (1.0f - 0.35f) + src * 0.35f;
-    const float o = t * (1.0f - h.cool);
-    __builtin_nontemporal_store(o > 0.0f ? o : 0.0f, h.tout + idx);
 }
 
 hipError_t launch_heat(aero_ctx* c, const aero_heat* hp) {
@@ -90,7 +119,7 @@ hipError_t launch_heat(aero_ctx* c, const aero_heat* hp) {
     h.tout = c->temps[c->tcur ^ 1];
     h.nx = c->grid.nx; h.ny = c->grid.ny; h.nz = c->grid.nz;
     h.dt = hp->dt; h.cool = hp->cool; h.diffuse = hp->diffuse;
-    const dim3 grid((c->grid.nx + 63) / 64, (c->grid.ny + 3) / 4, c->grid.nz);
+    const dim3 grid((c->grid.nx + 63) / 64, (c->grid.ny + 3) / 4, (c->grid.nz + kHeatCells - 1) / kHeatCells);
     hipLaunchKernelGGL(k_heat, grid, dim3(256), 0, c->stream, h);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) c->tcur ^= 1;
