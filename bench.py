@@ -2,7 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload f1|cube|large]
                     [--storage f32|f16s|bf16s] [--math fast|exact] [--no-cpu]
-                    [--no-aeromap] [--no-surrogate] [--no-flow] [--no-wings]
+                    [--no-aeromap] [--no-surrogate] [--no-flow] [--no-wings] [--no-slabs]
 
 Workload (default "f1"): the generic F1 body, 320x96x96 half tunnel at 4.5 cm
 cells, FP32 populations, D3Q19 regularized + Smagorinsky, moving floor, sponge
@@ -223,6 +223,55 @@ def wing_loads_bench(calls: int = 20, cpu: bool = True) -> dict:
     return out
 
 
+def slab_share_bench(parts: int = 8, steps: int = 200) -> dict:
+    """One GPU's share of the large tunnel (768x224x192, FP16S) split 8 ways along x
+    (slabs of 96x224x192), y (768x28x192) or z (768x224x24).  Each slab holds its own
+    part of the car and is stepped as a tunnel of its own: its boundary rules on the
+    two cut faces stand in for the halo planes a split run would fill from the
+    neighbouring slabs.  Each slab is autotuned and timed by the step stream's HIP
+    events; the slowest slab sets the pace of a split step.  A cut face carries the 5
+    populations that cross it, in FP16."""
+    import numpy as np
+
+    import aerolbm
+    from aerolbm import workloads as W
+
+    wl = W.large()
+    bpu = bytes_per_update("f16s")
+    out = {"tunnel": "large 768x224x192, f16s", "steps_timed": steps}
+    for axis in range(3):
+        dims = list(wl.dims)
+        dims[axis] //= parts
+        cells = dims[0] * dims[1] * dims[2]
+        face = [dims[1] * dims[2], dims[0] * dims[2], dims[0] * dims[1]][axis]
+        rows = []
+        for k in range(parts):
+            origin = list(wl.origin)
+            origin[axis] += k * dims[axis] * wl.h
+            e = aerolbm.HipAeroEngine(dims=tuple(dims), h=wl.h, origin=tuple(origin), u_lb=wl.u_lb, tau0=wl.tau0,
+                                      smag=wl.smag, storage="f16s", math="fast")
+            e.set_shape(wl.shape, prims=wl.prims)
+            solid_frac = float(np.count_nonzero(e.flags()[0])) / cells
+            e.reset()
+            e.autotune(steps_per_shape=10)
+            e.step(20)
+            e.sync()
+            e.profile_begin(max_calls=steps // 8 + 1)
+            for _ in range(steps // 8):
+                e.step(8)
+            e.sync()
+            ms, n = e.profile_end()
+            e.close()
+            us = ms * 1e3 / max(n, 1)
+            rows.append({"slab": k, "solid_frac": round(solid_frac, 4), "us_per_step": round(us, 2),
+                         "hbm_frac": round(cells * bpu / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)})
+            print(f"[bench] {'xyz'[axis]}-slab {k}: {us:.1f} us/step", file=sys.stderr, flush=True)
+        out["xyz"[axis]] = {"slab": "x".join(map(str, dims)), "cells_per_slab": cells, "slabs": rows,
+                            "slowest_us_per_step": max(r["us_per_step"] for r in rows),
+                            "face_bytes_per_step": 5 * face * 2}
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,10 +285,11 @@ def main() -> None:
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--no-flow", action="store_true")
     ap.add_argument("--no-wings", action="store_true")
+    ap.add_argument("--no-slabs", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(the tunnel would shard as x-slabs with one halo exchange per step; see DESIGN.md)",
+              f"(the tunnel would shard as y-slabs with one halo exchange per step)",
               file=sys.stderr)
         sys.exit(2)
 
@@ -345,6 +395,9 @@ def main() -> None:
     if not args.no_wings:
         print("[bench] wing loads (vortex lattice, 50 lattices per call)", file=sys.stderr, flush=True)
         result["wing_loads"] = wing_loads_bench(cpu=not args.no_cpu)
+    if not args.no_slabs:
+        print("[bench] one GPU's share of the large tunnel: 8 slabs along x, y and z", file=sys.stderr, flush=True)
+        result["slab_share"] = slab_share_bench()
     if not args.no_cpu:
         print("[bench] CPU baseline sample", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(wl)

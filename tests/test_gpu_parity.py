@@ -241,6 +241,53 @@ def test_storage_error_vs_fp64(storage, bound, math):
     assert ferr < 0.01
 
 
+@pytest.fixture(scope="module")
+def large_crop_fp64():
+    """The large workload's lattice (h = 2 cm, tau0 = 0.5005, Cs = 0.12, u = 0.05) and its
+    generic F1 body on a 128^3 crop that holds the front wing, the nose and the front
+    wheel (x from -0.9 m), flags from the GPU voxelizer, and 100 steps of the FP64 oracle
+    (OpenMP build) on them."""
+    wl = W.large()
+    dims, origin, steps = (128, 128, 128), (-0.9, 0.0, 0.0), 100
+    e = engine_for(dims, h=wl.h, origin=origin, u=wl.u_lb, tau0=wl.tau0, smag=wl.smag, storage="f32", math="fast")
+    e.set_shape(wl.shape, prims=wl.prims)
+    solid, emit = e.flags()
+    e.close()
+    assert np.bincount(solid, minlength=4)[1:3].min() > 1000  # body and front-wing cells present
+    g = orc.grid(dims, wl.h, origin)
+    sim = orc.OSim(wl.u_lb, wl.tau0, wl.smag, 0.0)
+    f64 = orc.LBMOracle("f64_omp")
+    f0, fields0 = f64.init(g, sim)
+    want, _, st = f64.run(g, sim, f0, solid, fields0, emit, steps)
+    return wl, dims, origin, steps, solid, emit, want, np.array(st.forces_d[:9])
+
+
+@pytest.mark.parametrize("storage,math,pbound", [("f32", "fast", 1e-5), ("f32", "exact", 1e-5),
+                                                 ("f16s", "fast", 3e-3), ("bf16s", "fast", 1.5e-2)])
+def test_large_workload_crop_error_vs_fp64(large_crop_fp64, storage, math, pbound):
+    """Each storage mode against FP64 on the large workload's parameters (the lowest
+    viscosity of the workloads): population error (of the largest population) and the
+    error of the force vector (x, y of the three groups) after 100 steps."""
+    wl, dims, origin, steps, solid, emit, want, fd = large_crop_fp64
+    e = engine_for(dims, h=wl.h, origin=origin, u=wl.u_lb, tau0=wl.tau0, smag=wl.smag, storage=storage, math=math)
+    e.set_shape(wl.shape, prims=wl.prims)
+    s2, e2 = e.flags()
+    assert np.array_equal(s2, solid) and np.array_equal(e2, emit)
+    e.reset()
+    e.step(steps)
+    got = e.populations()
+    perr = float(np.abs(got - want).max() / np.abs(want).max())
+    prel = rel_err(got, want)
+    d, _, n = e.read_forces_raw()
+    e.close()
+    ferr = float(np.linalg.norm(d - fd) / np.linalg.norm(fd))
+    print(f"large crop {storage}/{math}: population err {perr:.2e} of max ({prel:.2e} max relative), "
+          f"force-vector err {ferr:.2e}", flush=True)
+    assert n == steps and np.isfinite(got).all()
+    assert perr < pbound
+    assert ferr < 0.01
+
+
 def test_forces_reset_and_step_count():
     e = engine_for((40, 16, 12), u=0.05)
     s = W.base_shape(aerolbm.ShapeKind.Scene)
