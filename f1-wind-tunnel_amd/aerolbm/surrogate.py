@@ -137,25 +137,35 @@ def load_dataset(path: Path = DATA) -> Dict[str, np.ndarray]:
     return out
 
 
+def _setup_batch(part: np.ndarray, storage: str, perturb: float, rng) -> AeroBatch:
+    """One AeroBatch of the setups in ``part``, voxelized and initialised (populations
+    scaled by 1 + perturb * N(0, 1) per value when perturb is set)."""
+    b = AeroBatch([f1_grid()] * len(part), storage=storage)
+    try:
+        b.set_launch_shape(0)
+        for e, s in zip(b.engines, part):
+            shape, prims = setup_scene(s)
+            e.set_shape(shape, prims=prims)
+            e.reset()
+            if perturb:
+                p = e.populations()
+                e.write_populations((p * (1 + perturb * rng.standard_normal(p.shape))).astype(np.float32))
+    except BaseException:
+        b.close()
+        raise
+    return b
+
+
 def run_lbm(setups: np.ndarray, settle: int, measure: int, chunk: int = 64, storage: str = "f16s",
             perturb: float = 0.0, seed: int = 0, log=None) -> Tuple[np.ndarray, float]:
     """Labels of every setup from batched LBM runs; returns (Y [n, 4], wall seconds)."""
-    g = f1_grid()
     Y = np.zeros((len(setups), N_OUT))
     rng = np.random.default_rng(seed)
     t0 = time.perf_counter()
     for c0 in range(0, len(setups), chunk):
         part = setups[c0:c0 + chunk]
-        b = AeroBatch([g] * len(part), storage=storage)
+        b = _setup_batch(part, storage, perturb, rng)
         try:
-            b.set_launch_shape(0)
-            for e, s in zip(b.engines, part):
-                shape, prims = setup_scene(s)
-                e.set_shape(shape, prims=prims)
-                e.reset()
-                if perturb:
-                    p = e.populations()
-                    e.write_populations((p * (1 + perturb * rng.standard_normal(p.shape))).astype(np.float32))
             b.run_graph(settle, 100)
             for e in b.engines:
                 e.read_forces_raw(reset=True)
@@ -168,6 +178,40 @@ def run_lbm(setups: np.ndarray, settle: int, measure: int, chunk: int = 64, stor
         if log:
             log(f"  LBM {c0 + len(part)}/{len(setups)} setups, {time.perf_counter() - t0:.1f} s")
     return Y, time.perf_counter() - t0
+
+
+def block_sums(setups: np.ndarray, settle: int, block: int, blocks: int, storage: str = "f16s",
+               perturb: float = 0.0, seed: int = 1) -> np.ndarray:
+    """Raw force sums [blocks, n, 9] of consecutive ``block``-step windows after
+    ``settle`` steps, all setups in one batch: the input of label_scatter()."""
+    b = _setup_batch(setups, storage, perturb, np.random.default_rng(seed))
+    out = np.zeros((blocks, len(setups), 9))
+    try:
+        b.run_graph(settle, 100)
+        for e in b.engines:
+            e.read_forces_raw(reset=True)
+        for k in range(blocks):
+            b.run_graph(block, 100)
+            for i, e in enumerate(b.engines):
+                d, _, n = e.read_forces_raw(reset=True)
+                if n != block:
+                    raise RuntimeError(f"force window of {n} steps, expected {block}")
+                out[k, i] = d
+    finally:
+        b.close()
+    return out
+
+
+def label_scatter(a: np.ndarray, b: np.ndarray, block: int, windows: Sequence[int]) -> Dict[int, np.ndarray]:
+    """Median |difference| of the four labels between two runs of the same setups
+    (block sums from block_sums()), for averaging windows of the first k blocks."""
+    g = f1_grid()
+    out = {}
+    for k in windows:
+        ya = np.array([coefficients(s, k * block, g["h"], g["u_lb"]) for s in a[:k].sum(0)])
+        yb = np.array([coefficients(s, k * block, g["h"], g["u_lb"]) for s in b[:k].sum(0)])
+        out[k * block] = np.median(np.abs(ya - yb), 0)
+    return out
 
 
 @dataclass
@@ -243,9 +287,21 @@ class Surrogate:
 def main() -> None:
     """python -m aerolbm.surrogate OUT.npz [n] [settle] [measure]: writes the training
     set (LHS setups and their LBM labels, plus repeat runs of the first 64 setups from
-    a perturbed start, which measure how reproducible the labels themselves are)."""
+    a perturbed start, which measure how reproducible the labels themselves are).
+
+    python -m aerolbm.surrogate --scatter: the label scatter against the averaging
+    window (the first 64 setups twice, the second run from a 1e-3 perturbed start,
+    forces read every 2400 steps up to 14400 after a 2400-step settle)."""
     import sys
 
+    if sys.argv[1] == "--scatter":
+        X = lhs_setups(2048, seed=0)[:64]
+        a = block_sums(X, 2400, 2400, 6)
+        b = block_sums(X, 2400, 2400, 6, perturb=1e-3)
+        for w, med in label_scatter(a, b, 2400, (1, 2, 4, 6)).items():
+            print(f"window {w:6d} steps: median |a - b| " +
+                  ", ".join(f"{k} {v:.4f}" for k, v in zip(OUT_NAMES, med)), flush=True)
+        return
     out = sys.argv[1]
     n = int(sys.argv[2]) if len(sys.argv) > 2 else 2048
     settle = int(sys.argv[3]) if len(sys.argv) > 3 else 2400
