@@ -61,6 +61,7 @@ struct StepArgs {
     void* fout;
     const uint8_t* flags;
     const uint32_t* fastmap;  // bit per 64-cell row segment: every cell plain interior fluid
+    const uint32_t* edgemap;  // bit per segment: not fast, but no solid and no near-solid cell
     uint32_t nseg;            // 64-cell segments per row, ceil(nx / 64)
     float4* fields;         // written when write_fields != 0
     float* prev_u;          // outlet plane (x = nx-1) velocities, 3 floats per (y, z)
@@ -104,7 +105,7 @@ struct aero_ctx {
     uint32_t* solid;
     float* emit;
     uint8_t* flags;
-    uint32_t* fastmap;      // see StepArgs::fastmap
+    uint32_t* fastmap;      // fast_words words of StepArgs::fastmap, then fast_words of edgemap
     uint32_t nseg, fast_words;
     float4* fields;
     float* prev_u;

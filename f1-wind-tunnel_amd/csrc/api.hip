@@ -152,7 +152,7 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
         {(void**)&c->solid, n * 4},
         {(void**)&c->emit, n * 4},
         {(void**)&c->flags, c->ns},
-        {(void**)&c->fastmap, (size_t)c->fast_words * 4},
+        {(void**)&c->fastmap, (size_t)c->fast_words * 8},  // fast bits, then edge bits
         {(void**)&c->fields, n * 16},
         {(void**)&c->prev_u, outlet * 4},
         {(void**)&c->temps[0], n * 4},

@@ -220,10 +220,12 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
     // y, z and x >> 6 are uniform across the wavefront: its segment index is scalar.
     const uint32_t seg = (uint32_t)__builtin_amdgcn_readfirstlane(
         (int)(((uint32_t)x >> 6) + a.nseg * ((uint32_t)y + (uint32_t)NY * (uint32_t)z)));
-    bool fast = false;
+    bool fast = false, edge = false;
     if (y < NY) {
         const uint32_t word = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.fastmap[seg >> 5]);
+        const uint32_t eword = (uint32_t)__builtin_amdgcn_readfirstlane((int)a.edgemap[seg >> 5]);
         fast = ((word >> (seg & 31u)) & 1u) != 0u;
+        edge = ((eword >> (seg & 31u)) & 1u) != 0u;
     }
     // Byte offset of population 0 of this lane's own cell; population I adds I * POPB.
     const uint32_t own = seg * SEGB + (uint32_t)lane * ESZ;
@@ -245,6 +247,54 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
     if (active) {
         const bool outlet = (x == NX - 1);
         const uint32_t pidx = ((uint32_t)y + (uint32_t)NY * (uint32_t)z) * 3u;
+        const bool flo_row = (y == 0), far_y = (y == NY - 1), far_z = (z == NZ - 1), mir_row = (z == 0);
+        // Byte offset of the source of direction I that follows from the cell's
+        // coordinates alone (wgsl.ts:203-219 without the solid test): the same lane of
+        // the segment (cy, cz) rows back, one lane over for c_x = +-1 (crossing into the
+        // neighbouring segment at lane 0 / lane 63); in the symmetry row z = 0 the mirrored
+        // population of this plane; on the moving floor the cell's own opposite
+        // population; for a constant-valued pull (inlet, outlet, far field) the cell's own
+        // value, which is replaced afterwards.  Unsigned arithmetic wraps modulo 2^32; an
+        // offset is used only when its source lies in the array.
+        const uint32_t lane_p = lane == 0 ? 63u * ESZ - SEGB : (uint32_t)(lane - 1) * ESZ;  // c_x = +1
+        const uint32_t lane_m = lane == 63 ? SEGB : (uint32_t)(lane + 1) * ESZ;              // c_x = -1
+        const uint32_t rowb = a.nseg * SEGB, planeb = a.nseg * (uint32_t)NY * SEGB;
+        const uint32_t seg_base = seg * SEGB;
+        auto coord_off = [&](auto ic) -> uint32_t {
+            constexpr int I = decltype(ic)::value;
+            constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
+            if constexpr (I == 0) {
+                return own;
+            } else {
+                uint32_t reg = seg_base + (cx == 1 ? lane_p : (cx == -1 ? lane_m : (uint32_t)lane * ESZ));
+                if constexpr (cy == 1) reg -= rowb;
+                if constexpr (cy == -1) reg += rowb;
+                if constexpr (cz == 1) reg = mir_row ? reg + (uint32_t)kZMIR[I] * POPB : reg - planeb + (uint32_t)I * POPB;
+                else if constexpr (cz == -1) reg += planeb + (uint32_t)I * POPB;
+                else reg += (uint32_t)I * POPB;
+                const uint32_t opp = own + (uint32_t)kOPP[I] * POPB;
+                const uint32_t dummy = own + (uint32_t)I * POPB;
+                bool constant = false, bounce = false;
+                if constexpr (cx == 1) constant = constant || (x == 0);
+                if constexpr (cx == -1) constant = constant || outlet;
+                if constexpr (cy == 1) bounce = !constant && flo_row;
+                bool farf = false;
+                if constexpr (cy == -1) farf = farf || far_y;
+                if constexpr (cz == -1) farf = farf || far_z;
+                if constexpr (cy != 1) constant = constant || farf;
+                else constant = constant || (farf && !flo_row);
+                return constant ? dummy : (bounce ? opp : reg);
+            }
+        };
+        float f[19];
+        if (edge) {
+            // No solid within one link of this segment: the loads go out before the
+            // cell flags (needed only for the floor-contact and duct bits) arrive.
+            for19([&](auto ic) {
+                constexpr int I = decltype(ic)::value;
+                f[I] = LD(coord_off(ic), 0u, wt(I));
+            });
+        }
         if (fl & FL_GROUP) {
             // Solid cell: fout_i = w_i, fields = 0 (wgsl.ts:182-186).
             for19([&](auto ic) {
@@ -269,7 +319,6 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                 else return feq_c<I>(wt(I), pux, puy, puz, puu15);
             };
 
-            float f[19];
             if (fast) {
                 // The source of direction I is the same lane of the segment (cy, cz) rows
                 // back, shifted by one lane for c_x = +-1 (crossing into the neighbouring
@@ -312,7 +361,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                 //       array (constant-valued pulls read the cell's own value);
                 //   (3) the 19 loads go out together, then constants, the floor term
                 //       and the momentum exchange are applied.
-                const bool flo_row = (y == 0), far_y = (y == NY - 1), far_z = (z == NZ - 1), mir_row = (z == 0);
+                // Edge segments (no solid within one link) issued their loads above.
                 uint32_t s1 = 0u, s2 = 0u, s3 = 0u;  // bit I: source of direction I is solid of group 1/2/3
                 auto special = [&](auto ic) -> bool {  // inlet/outlet/floor/far field: no solid test
                     constexpr int I = decltype(ic)::value;
@@ -331,7 +380,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                     if constexpr (cz == 1) return mir_row ? cx + cy * NX : cx + cy * NX + sxy;
                     else return cx + cy * NX + cz * sxy;
                 };
-                if (__builtin_amdgcn_ballot_w64(near) != 0) {
+                if (!edge && __builtin_amdgcn_ballot_w64(near) != 0) {
                     uint32_t nb[19];
                     for19([&](auto ic) {
                         constexpr int I = decltype(ic)::value;
@@ -352,43 +401,16 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                     });
                 }
                 const uint32_t sany = s1 | s2 | s3;
-                uint32_t voff[19];
-                // byte offset of population p of cell (sx, sy, sz) (only used in range)
-                auto aoff = [&](int sx, int sy, int sz, uint32_t p) -> uint32_t {
-                    const uint32_t sseg = (uint32_t)(sx >> 6) + a.nseg * ((uint32_t)sy + (uint32_t)NY * (uint32_t)sz);
-                    return sseg * SEGB + p * POPB + (uint32_t)(sx & 63) * ESZ;
-                };
-                for19([&](auto ic) {
-                    constexpr int I = decltype(ic)::value;
-                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
-                    if constexpr (I == 0) {
-                        voff[0] = own;
-                    } else {
-                        // plane of a regular or mirrored pull
-                        uint32_t plane = (uint32_t)I;
-                        if constexpr (cz == 1) plane = mir_row ? (uint32_t)kZMIR[I] : (uint32_t)I;
-                        int sz = z - cz;
-                        if constexpr (cz == 1) sz = mir_row ? z : z - 1;
-                        const uint32_t reg = aoff(x - cx, y - cy, sz, plane);
+                if (!edge) {
+                    // A solid source (never a special one: the ballot above tests only
+                    // non-special links) bounces back the cell's own opposite population.
+                    for19([&](auto ic) {
+                        constexpr int I = decltype(ic)::value;
                         const uint32_t opp = own + (uint32_t)kOPP[I] * POPB;
-                        const uint32_t dummy = own + (uint32_t)I * POPB;
-                        bool constant = false, bounce = false;
-                        if constexpr (cx == 1) constant = constant || (x == 0);
-                        if constexpr (cx == -1) constant = constant || outlet;
-                        if constexpr (cy == 1) bounce = !constant && flo_row;
-                        bool farf = false;
-                        if constexpr (cy == -1) farf = farf || far_y;
-                        if constexpr (cz == -1) farf = farf || far_z;
-                        if constexpr (cy != 1) constant = constant || farf;
-                        else constant = constant || (farf && !flo_row);
-                        bounce = bounce || (((sany >> I) & 1u) != 0u);
-                        voff[I] = constant ? dummy : (bounce ? opp : reg);
-                    }
-                });
-                for19([&](auto ic) {
-                    constexpr int I = decltype(ic)::value;
-                    f[I] = LD(voff[I], 0u, wt(kOPP[I]) /* w_opp == w_I */);
-                });
+                        const uint32_t off = (((sany >> I) & 1u) != 0u) ? opp : coord_off(ic);
+                        f[I] = LD(off, 0u, wt(kOPP[I]) /* w_opp == w_I */);
+                    });
+                }
                 for19([&](auto ic) {
                     constexpr int I = decltype(ic)::value;
                     constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
@@ -656,9 +678,10 @@ k_step_persist(StepArgs a) {
 // Batched form: one launch steps every tile of every tunnel of a batch.  Tiles are
 // numbered instance after instance; a block finds its instance by binary search
 // in the (wave-uniform) tile offsets and runs the same tile body with that
-// instance's constants.  All instances share the ping-pong parity.
+// instance's constants.  All instances share the ping-pong parity.  Seven waves per
+// SIMD (at most 72 VGPRs), as the single-tunnel kernel reaches without a bound.
 template <int S, bool EXACT, int TX, bool SWZ>
-__global__ void __launch_bounds__(kBlock) k_step_batch(BatchArgs b) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) k_step_batch(BatchArgs b) {
     const uint32_t g = SWZ ? xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
     uint32_t lo = 0, hi = b.ninst;
     while (hi - lo > 1) {
@@ -806,32 +829,41 @@ __global__ void __launch_bounds__(256) k_unpack(const void* in, float* __restric
     for (int i = 0; i < 19; i++) out[(uint64_t)i * n + idx] = ST::ld((const T*)in + seg_elem(idx, i, nx, nseg), wt(i));
 }
 
-// Fast-segment map: bit s (s = k + nseg*(y + ny*z)) is set when the 64-cell row
-// segment x in [64k, 64k+63] of row (y, z) lies off the floor, roof, side and symmetry
-// rows and none of its cells is solid, next to a solid, or a duct cell.  A wavefront
-// covers exactly one segment in every launch shape, so it reads one bit instead of
-// 64 flag bytes to choose the plain pull.
+// Segment maps, bit s (s = k + nseg*(y + ny*z)) for the 64-cell row segment x in
+// [64k, 64k+63] of row (y, z).  A wavefront covers exactly one segment in every launch
+// shape, so it reads two bits instead of 64 flag bytes to choose its pull:
+//   fast (map[0, words)): off the floor, roof, side and symmetry rows, and no cell is
+//     solid, next to a solid, or a duct cell;
+//   edge (map[words, 2 words)): not fast, and no cell is solid or next to a solid, so
+//     every source follows from the cell's coordinates alone.
 __global__ void __launch_bounds__(256) k_fastmap(const uint8_t* __restrict__ flags, uint32_t* __restrict__ map,
                                                  uint32_t words, uint32_t nseg, uint32_t nx, uint32_t ny, uint32_t nz) {
     // One thread per segment; a wavefront's 64 segment bits are words 2w and 2w + 1.
     const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint64_t total = (uint64_t)nseg * ny * nz;
-    bool fast = false;
+    bool fast = false, clear = false;
     if (s < total) {
         const uint32_t k = (uint32_t)(s % nseg);
         const uint32_t y = (uint32_t)((s / nseg) % ny);
         const uint32_t z = (uint32_t)(s / ((uint64_t)nseg * ny));
-        fast = y >= 1u && y + 2u <= ny && z >= 1u && z + 2u <= nz;
+        const bool inner = y >= 1u && y + 2u <= ny && z >= 1u && z + 2u <= nz;
         const uint32_t x1 = (k * 64u + 64u < nx) ? k * 64u + 64u : nx;
         const uint64_t row = (uint64_t)nx * ((uint64_t)y + (uint64_t)ny * z);
-        for (uint32_t x = k * 64u; fast && x < x1; x++)
-            if (flags[row + x] & (FL_GROUP | FL_NEAR | FL_DUCT)) fast = false;
+        bool duct = false;
+        clear = true;
+        for (uint32_t x = k * 64u; clear && x < x1; x++) {
+            const uint8_t f = flags[row + x];
+            if (f & (FL_GROUP | FL_NEAR)) clear = false;
+            if (f & FL_DUCT) duct = true;
+        }
+        fast = clear && inner && !duct;
     }
     const uint64_t bits = __builtin_amdgcn_ballot_w64(fast);
+    const uint64_t ebits = __builtin_amdgcn_ballot_w64(clear && !fast);
     const uint64_t w0 = s >> 5;  // s is a multiple of 64 in lane 0
     if ((threadIdx.x & 63) == 0) {
-        if (w0 < words) map[w0] = (uint32_t)bits;
-        if (w0 + 1 < words) map[w0 + 1] = (uint32_t)(bits >> 32);
+        if (w0 < words) { map[w0] = (uint32_t)bits; map[words + w0] = (uint32_t)ebits; }
+        if (w0 + 1 < words) { map[w0 + 1] = (uint32_t)(bits >> 32); map[words + w0 + 1] = (uint32_t)(ebits >> 32); }
     }
 }
 
@@ -857,6 +889,7 @@ void fill_step_consts(const aero_ctx* c, StepArgs* a) {
     }
     a->flags = c->flags;
     a->fastmap = c->fastmap;
+    a->edgemap = c->fastmap + c->fast_words;
     a->nseg = c->nseg;
     a->fields = c->fields;
     a->prev_u = c->prev_u;
