@@ -1,5 +1,5 @@
 """Setup -> force-coefficient surrogate: an MLP ensemble trained on batched LBM runs of
-the generic F1 body, with the LBM as fallback and judge.
+the generic F1 body, with the LBM as fallback and as the check of its error.
 
 Inputs (16, the setup of one run):
     0 front ride height (m, 0.020-0.060)        1 rear ride height (m, 0.040-0.120)

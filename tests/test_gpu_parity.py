@@ -1,7 +1,7 @@
 """GPU parity: libaero_lbm.so (through its C ABI) against the CPU oracle on the same
 seeded inputs.  FP32 storage is expected to reproduce the f32 restatement of the
 reference kernels bit for bit (the tolerance written in each test is the
-north-star bound, 1e-5 relative); FP16/BF16-shifted storage is bounded against the
+parity bound, 1e-5 relative); FP16/BF16-shifted storage is bounded against the
 FP64 oracle."""
 import math
 
