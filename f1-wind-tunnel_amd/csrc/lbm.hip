@@ -230,6 +230,9 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
     // Byte offset of population 0 of this lane's own cell; population I adds I * POPB.
     const uint32_t own = seg * SEGB + (uint32_t)lane * ESZ;
     const uint32_t fl = (!fast && active) ? (uint32_t)*at(a.flags, idx) : 0u;
+    // tau0 + sponge ramps of this column (wgsl.ts:259-260), fetched with the populations
+    // so that its latency is not paid again after the moments.
+    const float tau_col = active ? *at(a.tau_x, (uint32_t)x) : 0.0f;
 
     auto LD = [&](uint32_t voff, uint32_t soff, float w) -> float {
         if constexpr (DEV) return ST::braw(rin, voff, soff);
@@ -509,7 +512,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                 // Smagorinsky relaxation time with the inlet/outlet sponge (wgsl.ts:258-262).
                 const float q = __builtin_sqrtf(((pxx * pxx + pyy * pyy) + pzz * pzz) +
                                                 2.0f * ((pxy * pxy + pxz * pxz) + pyz * pyz));
-                const float tau0 = *at(a.tau_x, (uint32_t)x);
+                const float tau0 = tau_col;
                 const float tau = 0.5f * (tau0 + __builtin_sqrtf(tau0 * tau0 + a.smag_c * q / rho));
                 const float omega = 1.0f / tau;
                 const float om45 = (1.0f - omega) * 4.5f;
@@ -583,7 +586,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                 const float q2 = __builtin_fmaf(pxx, pxx, __builtin_fmaf(pyy, pyy, pzz * pzz)) +
                                  2.0f * __builtin_fmaf(pxy, pxy, __builtin_fmaf(pxz, pxz, pyz * pyz));
                 const float q = __builtin_amdgcn_sqrtf(q2);
-                const float tau0 = *at(a.tau_x, (uint32_t)x);
+                const float tau0 = tau_col;
                 const float tau = 0.5f * (tau0 + __builtin_amdgcn_sqrtf(__builtin_fmaf(tau0, tau0, a.smag_c * q * inv)));
                 const float k1 = 1.0f - __builtin_amdgcn_rcpf(tau);  // 1 - omega
                 // M = j u + (1 - omega) Pi;  A = rho - 1.5 (j.u + (1 - omega) tr Pi)
