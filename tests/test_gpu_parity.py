@@ -95,6 +95,81 @@ def test_step_matches_f32_oracle_mixed_geometry(steps, math):
     assert e.guard_count() == st.guard_hits
 
 
+def segment_kinds(solid, emit, dims):
+    """Host restatement of the kernel's segment classes: fast (inner row, no solid,
+    near-solid or duct cell), edge (not fast, no solid or near-solid cell), general."""
+    nx, ny, nz = dims
+    S = solid.reshape(nz, ny, nx) != 0
+    P = np.pad(S, 1)
+    near = np.zeros_like(S)
+    for dz in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dx in (-1, 0, 1):
+                if 0 < abs(dx) + abs(dy) + abs(dz) <= 2:
+                    near |= P[1 + dz:1 + dz + nz, 1 + dy:1 + dy + ny, 1 + dx:1 + dx + nx]
+    nseg = (nx + 63) // 64
+    def per_seg(m):
+        pad = np.zeros((nz, ny, nseg * 64), bool)
+        pad[:, :, :nx] = m
+        return pad.reshape(nz, ny, nseg, 64).any(-1)
+    blocked = per_seg(S | near)
+    duct = per_seg(emit.reshape(nz, ny, nx) < 0)
+    inner = np.zeros((nz, ny, 1), bool)
+    inner[1:-1, 1:-1] = True
+    fast = ~blocked & ~duct & inner
+    return fast, ~blocked & ~fast, blocked
+
+
+@pytest.mark.parametrize("math", ["exact", "fast"])
+@pytest.mark.parametrize("steps", [1, 60])
+def test_edge_segments_with_ducts_and_boundaries(steps, math):
+    """Three segments per row: solids in the middle one, duct cells (emit < 0) on the
+    floor, roof, symmetry and side rows far from any solid, so that boundary-row
+    segments with no solid nearby (their loads issued before their flag bytes arrive)
+    carry the floor, far-field, mirror and duct rules together."""
+    rng = np.random.default_rng(12)
+    dims = (160, 20, 12)
+    nx, ny, nz = dims
+    s = np.zeros((nz, ny, nx), np.uint32)
+    em = np.zeros((nz, ny, nx), np.float32)
+    s[0:5, 0:4, 70:80] = 1       # on the floor and the symmetry plane
+    s[4:8, 8:12, 84:90] = 2
+    s[6:9, 14:16, 100:104] = 3
+    em[0, 0, 140:150] = -1.0     # duct cells on the floor row of the symmetry plane
+    em[5, 0, 10:20] = -1.0       # floor row
+    em[nz - 1, 7, 130:140] = -1.0  # side row
+    em[3, ny - 1, 20:30] = -1.0  # roof row
+    em[0, 9, 145:155] = -1.0     # symmetry row
+    em[2, 4, 80:82] = 300.0      # a heat source inside a solid
+    solid, emit = s.reshape(-1), em.reshape(-1)
+    fast, edge, general = segment_kinds(solid, emit, dims)
+    duct_seg = np.pad(em < 0, ((0, 0), (0, 0), (0, 3 * 64 - nx))).reshape(nz, ny, 3, 64).any(-1)
+    assert fast.any() and general.any() and (edge & duct_seg).sum() >= 4
+    e = engine_for(dims, u=0.07, math=math)
+    e.set_flags(solid, emit)
+    e.reset()
+    f, fields0 = perturbed_init(e, rng)
+    e.write_populations(f)
+    e.step(steps)
+    got = e.populations()
+    want, wfields, st = F32.run(ogrid(e), osim(e), f, solid, fields0, emit, steps)
+    print(f"edge segments {int(edge.sum())} ({int((edge & duct_seg).sum())} with ducts), {math} steps={steps}: "
+          f"max rel err {rel_err(got, want):.3e}", flush=True)
+    assert rel_err(got, want) <= REL
+    d, q, n = e.read_forces_raw()
+    od = np.array(st.forces_d)
+    gf = e.fields()
+    assert rel_err(gf[:, 3], wfields[:, 3]) <= REL
+    assert np.abs(gf[:, :3] - wfields[:, :3]).max() <= (1e-7 if math == "exact" else 2e-6)
+    if math == "exact":
+        assert np.array_equal(got, want)
+        assert np.array_equal(q, np.array(st.forces_q)[:9])
+    else:
+        assert np.abs(d - od).max() <= 1e-5 * np.abs(od).max()
+    assert n == steps and e.guard_count() == st.guard_hits
+    e.close()
+
+
 def test_init_matches_oracle():
     e = engine_for((40, 12, 9), u=0.0925)
     e.reset()
