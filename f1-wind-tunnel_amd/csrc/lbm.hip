@@ -216,7 +216,11 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
     // The choice is one bit of the fast-segment map (a wave covers one 64-cell row
     // segment), read with a wave-uniform address; per-cell flags are loaded only by
     // wavefronts that take the general path.
-    const bool first_tile = (tx == 0), last_tile = (tx == a.tiles_x - 1);
+    // The wavefront holding the inlet column x = 0 and the one holding the outlet column
+    // x = nx - 1 (a wavefront covers one 64-cell segment, so this is wave-uniform; a tile
+    // wider than 64 cells spans several segments, and only these two patch values).
+    const uint32_t wseg_x = (uint32_t)__builtin_amdgcn_readfirstlane((int)((uint32_t)x >> 6));
+    const bool inlet_seg = (wseg_x == 0u), outlet_seg = (wseg_x == ((uint32_t)(NX - 1) >> 6));
     // y, z and x >> 6 are uniform across the wavefront: its segment index is scalar.
     const uint32_t seg = (uint32_t)__builtin_amdgcn_readfirstlane(
         (int)(((uint32_t)x >> 6) + a.nseg * ((uint32_t)y + (uint32_t)NY * (uint32_t)z)));
@@ -311,7 +315,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
             const bool duct = (fl & FL_DUCT) != 0;
             const bool floor0 = (fl & FL_FLOOR0) != 0;
             float pux = 0.f, puy = 0.f, puz = 0.f, puu15 = 0.f;
-            if (last_tile && outlet) {
+            if (outlet_seg && outlet) {
                 pux = a.prev_u[pidx]; puy = a.prev_u[pidx + 1]; puz = a.prev_u[pidx + 2];
                 puu15 = 1.5f * dot3(pux, puy, puz);
             }
@@ -340,13 +344,13 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                     const int32_t src = base - cy * row - cz * plane + I * (int32_t)POPB;
                     f[I] = LD((uint32_t)(src + lo), 0u, wt(I));
                 });
-                if (first_tile) {  // inlet lane x = 0 (wgsl.ts:205)
+                if (inlet_seg) {  // inlet lane x = 0 (wgsl.ts:205)
                     for19([&](auto ic) {
                         constexpr int I = decltype(ic)::value;
                         if constexpr (kCX[I] == 1) f[I] = (x == 0) ? (DEV ? a.feq_in_dev[I] : a.feq_in[I]) : f[I];
                     });
                 }
-                if (last_tile) {  // outlet lane x = nx-1 (wgsl.ts:207)
+                if (outlet_seg) {  // outlet lane x = nx-1 (wgsl.ts:207)
                     for19([&](auto ic) {
                         constexpr int I = decltype(ic)::value;
                         if constexpr (kCX[I] == -1) {
@@ -624,7 +628,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                 });
             }
             if (write_fields) a.fields[idx] = make_float4(ux, uy, uz, rho);
-            if (last_tile && outlet) { a.prev_u[pidx] = ux; a.prev_u[pidx + 1] = uy; a.prev_u[pidx + 2] = uz; }
+            if (outlet_seg && outlet) { a.prev_u[pidx] = ux; a.prev_u[pidx + 1] = uy; a.prev_u[pidx + 2] = uz; }
         }
     }
 
