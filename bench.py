@@ -41,7 +41,7 @@ def bytes_per_update(storage: str) -> int:
     return 2 * 19 * s + 1
 
 
-def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
+def cpu_baseline(wl, budget_s: float = 15.0) -> dict:
     """FP32 restatement of the reference kernels (OpenMP C), timed on a bounded
     sample of the same workload."""
     import numpy as np
@@ -55,9 +55,9 @@ def cpu_baseline(wl, budget_s: float = 12.0) -> dict:
     solid, emit, _ = sdf.voxelize(g, wl.shape, prims=wl.prims)
     f, fields = ref.init(g, sim)
     t0 = time.perf_counter()
-    f, fields, _ = ref.run(g, sim, f, solid, fields, emit, 2)
+    f, fields, _ = ref.run(g, sim, f, solid, fields, emit, 4)
     dt = time.perf_counter() - t0
-    steps = max(2, min(400, int(budget_s / max(dt / 2, 1e-6))))
+    steps = max(2, min(2000, int(budget_s / max(dt / 4, 1e-6))))
     t0 = time.perf_counter()
     ref.run(g, sim, f, solid, fields, emit, steps)
     dt = time.perf_counter() - t0

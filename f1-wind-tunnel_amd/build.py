@@ -72,6 +72,12 @@ def build(verbose: bool = False) -> Path:
     if verbose:
         print(f"built {out}")
     build_addon(verbose)
+    # The HBM counter calibration program (perf/hbm_copy_calib.hip) used by the profiles.
+    calib = LIB / "hbm_copy_calib"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-o", str(calib), str(PKG / "perf" / "hbm_copy_calib.hip")])
+    if verbose:
+        print(f"built {calib}")
     return out
 
 
