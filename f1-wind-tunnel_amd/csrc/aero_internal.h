@@ -82,7 +82,7 @@ struct StepArgs {
 // Arguments of the batched stream-collide kernel.
 struct BatchArgs {
     const StepArgs* tab;    // per instance (fin = buffer 0, fout = buffer 1)
-    const uint32_t* begin;  // ninst + 1 tile offsets
+    const uint2* tilemap;   // per tile of the launch: (instance within tab, tile within the instance)
     uint32_t ninst;
     int parity;             // 1: every instance reads buffer 1 and writes buffer 0
     int write_fields;

@@ -26,8 +26,11 @@ struct aero_batch {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     StepArgs* d_tab = nullptr;
-    uint32_t* d_begin = nullptr;     // per group: count + 1 tile offsets
-    struct Group { uint32_t first, count, begin_off, tiles; };
+    uint2* d_tilemap = nullptr;      // per tile of each group's launch: (instance in group, tile in instance)
+    size_t tilemap_cap = 0;          // entries: the most tiles of any batch launch shape
+    int map_shape = -1;              // shape and grouping the uploaded tile map was built for
+    uint64_t map_group_bytes = 0;
+    struct Group { uint32_t first, count, map_off, tiles; };
     std::vector<Group> groups;
     uint64_t group_bytes = 0;        // 0: one group holding every instance
     // cached graph of `graph_steps` fused steps (even, so parity is restored)
@@ -60,15 +63,15 @@ void drop_graph(aero_batch* b) {
     b->graph_shape = -1;
 }
 
-// Uploads the instance table and the per-group tile offsets for the current shape.
-// Every instance must be at the same ping-pong parity; returns that parity in *parity.
+// Uploads the instance table and, when the shape or the grouping changed, the tile map
+// for the current shape.  Every instance must be at the same ping-pong parity; returns
+// that parity in *parity.
 int prepare(aero_batch* b, int* parity) {
     const int p0 = b->inst[0]->cur;
     const size_t n = b->inst.size();
     std::vector<StepArgs> tab(n);
-    std::vector<uint32_t> begin;
     b->groups.clear();
-    uint64_t bytes = 0;
+    uint64_t bytes = 0, total = 0;
     for (size_t i = 0; i < n; i++) {
         aero_ctx* c = b->inst[i];
         if (c->cur != p0) return bfail(b, AERO_E_STATE, "aero_batch: instances are at different step parities");
@@ -81,20 +84,29 @@ int prepare(aero_batch* b, int* parity) {
         tab[i] = a;
         const uint64_t ib = 2ull * c->pop_bytes;
         if (b->groups.empty() || (b->group_bytes && bytes + ib > b->group_bytes)) {
-            if (!b->groups.empty()) begin.push_back(b->groups.back().tiles);
-            b->groups.push_back({(uint32_t)i, 0u, (uint32_t)begin.size(), 0u});
+            b->groups.push_back({(uint32_t)i, 0u, (uint32_t)total, 0u});
             bytes = 0;
         }
         aero_batch::Group& g = b->groups.back();
-        begin.push_back(g.tiles);
         g.tiles += a.ntiles;
         g.count++;
+        total += a.ntiles;
         bytes += ib;
     }
-    begin.push_back(b->groups.back().tiles);
+    if (total > b->tilemap_cap) return bfail(b, AERO_E_STATE, "aero_batch: tile map smaller than the launch");
     BCHK(b, hipMemcpyAsync(b->d_tab, tab.data(), tab.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
-    BCHK(b, hipMemcpyAsync(b->d_begin, begin.data(), begin.size() * sizeof(uint32_t), hipMemcpyHostToDevice, b->stream));
+    std::vector<uint2> map;
+    if (b->map_shape != b->shape || b->map_group_bytes != b->group_bytes) {
+        map.reserve(total);
+        for (const aero_batch::Group& g : b->groups)
+            for (uint32_t i = 0; i < g.count; i++)
+                for (uint32_t t = 0; t < tab[g.first + i].ntiles; t++) map.push_back(make_uint2(i, t));
+        BCHK(b, hipMemcpyAsync(b->d_tilemap, map.data(), map.size() * sizeof(uint2), hipMemcpyHostToDevice,
+                               b->stream));
+    }
     BCHK(b, hipStreamSynchronize(b->stream));  // the host tables go out of scope here
+    b->map_shape = b->shape;
+    b->map_group_bytes = b->group_bytes;
     *parity = p0;
     return AERO_OK;
 }
@@ -102,7 +114,7 @@ int prepare(aero_batch* b, int* parity) {
 BatchArgs args(aero_batch* b, const aero_batch::Group& g, int parity, int write_fields) {
     BatchArgs a{};
     a.tab = b->d_tab + g.first;
-    a.begin = b->d_begin + g.begin_off;
+    a.tilemap = b->d_tilemap + g.map_off;
     a.ninst = g.count;
     a.parity = parity;
     a.write_fields = write_fields;
@@ -152,15 +164,27 @@ int aero_batch_create(uint32_t n, const aero_grid* grids, const aero_sim* sims, 
     uint64_t cells = 0;
     for (aero_ctx* c : b->inst) cells += c->grid.n;
     b->shape = cells > 8000000ull ? 6 : 0;
+    // The tile map holds one entry per tile of a launch; its size is that of the launch
+    // shape with the most tiles, so it is allocated once and never moves under a graph.
+    for (int s = 0; s < kNumShapes; s++) {
+        if (!batch_shape_ok(s)) continue;
+        size_t tiles = 0;
+        for (aero_ctx* c : b->inst) {
+            uint32_t tx, ty, nt;
+            tile_geometry(c, s, &tx, &ty, &nt);
+            tiles += nt;
+        }
+        if (tiles > b->tilemap_cap) b->tilemap_cap = tiles;
+    }
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreate(&b->ev0);
     if (e == hipSuccess) e = hipEventCreate(&b->ev1);
     if (e == hipSuccess) e = hipMalloc(&b->d_tab, n * sizeof(StepArgs));
-    if (e == hipSuccess) e = hipMalloc(&b->d_begin, (2 * (size_t)n + 1) * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMalloc(&b->d_tilemap, b->tilemap_cap * sizeof(uint2));
     if (e != hipSuccess) {
         for (aero_ctx* x : b->inst) aero_destroy(x);
         if (b->d_tab) (void)hipFree(b->d_tab);
-        if (b->d_begin) (void)hipFree(b->d_begin);
+        if (b->d_tilemap) (void)hipFree(b->d_tilemap);
         if (b->stream) (void)hipStreamDestroy(b->stream);
         delete b;
         return e == hipErrorOutOfMemory ? AERO_E_NOMEM : AERO_E_HIP;
@@ -176,7 +200,7 @@ int aero_batch_destroy(aero_batch* b) {
     drop_graph(b);
     for (aero_ctx* c : b->inst) aero_destroy(c);
     if (b->d_tab) (void)hipFree(b->d_tab);
-    if (b->d_begin) (void)hipFree(b->d_begin);
+    if (b->d_tilemap) (void)hipFree(b->d_tilemap);
     if (b->ev0) (void)hipEventDestroy(b->ev0);
     if (b->ev1) (void)hipEventDestroy(b->ev1);
     if (b->stream) (void)hipStreamDestroy(b->stream);

@@ -683,22 +683,18 @@ k_step_persist(StepArgs a) {
 }
 
 // Batched form: one launch steps every tile of every tunnel of a batch.  Tiles are
-// numbered instance after instance; a block finds its instance by binary search
-// in the (wave-uniform) tile offsets and runs the same tile body with that
-// instance's constants.  All instances share the ping-pong parity.  Seven waves per
-// SIMD (at most 72 VGPRs), as the single-tunnel kernel reaches without a bound.
+// numbered instance after instance; a block reads its (instance, tile) pair from the
+// tile map with one scalar load (a search over the instances' tile offsets would be a
+// chain of dependent scalar loads ahead of every block's first population load) and
+// runs the same tile body with that instance's constants.  All instances share the
+// ping-pong parity.  Seven waves per SIMD (at most 72 VGPRs), as the single-tunnel
+// kernel reaches without a bound.
 template <int S, bool EXACT, int TX, bool SWZ>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) k_step_batch(BatchArgs b) {
     const uint32_t g = SWZ ? xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
-    uint32_t lo = 0, hi = b.ninst;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (b.begin[mid] <= g) lo = mid;
-        else hi = mid;
-    }
-    const StepArgs& a = b.tab[lo];
-    step_tile<S, EXACT, TX>(a, g - b.begin[lo], b.parity ? a.fout : a.fin, b.parity ? (void*)a.fin : a.fout,
-                            b.write_fields);
+    const uint2 m = b.tilemap[g];
+    const StepArgs& a = b.tab[m.x];
+    step_tile<S, EXACT, TX>(a, m.y, b.parity ? a.fout : a.fin, b.parity ? (void*)a.fin : a.fout, b.write_fields);
 }
 
 // Element of population i of cell idx in the segment-major population layout.

@@ -267,3 +267,28 @@ def test_instance_groups_give_identical_results():
     for other in runs[1:]:
         for (p0, (d0, q0, n0)), (p1, (d1, q1, n1)) in zip(runs[0], other):
             assert np.array_equal(p0, p1) and np.array_equal(d0, d1) and np.array_equal(q0, q1) and n0 == n1
+
+
+def test_shape_and_group_changes_between_graph_runs():
+    """The per-tile (instance, tile) map is rebuilt whenever the launch shape or the
+    instance grouping changes: graph runs interleaved with shape and group switches
+    (and a return to an earlier shape) stay bit-identical to the tunnels stepped one
+    by one."""
+    print("shape/group switches", flush=True)
+    b = AeroBatch(SPECS, storage="f16s")
+    ref = singles("f16s", "fast")
+    setup(b.engines)
+    setup(ref)
+    total = 0
+    for shape, gb, n in [(5, 0, 8), (0, 0, 6), (7, 1, 4), (5, 0, 10), (4, 2 * 2 * 64 * 24 * 16 * 19 * 2 + 1, 6)]:
+        b.set_group_bytes(gb)
+        b.set_launch_shape(shape)
+        b.run_graph(n, 2)
+        total += n
+    for e in ref:
+        e.step(total)
+    for eb, er in zip(b.engines, ref):
+        same_state(eb, er)
+    b.close()
+    for e in ref:
+        e.close()
