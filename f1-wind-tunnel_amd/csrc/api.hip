@@ -117,6 +117,11 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
         return fail(nullptr, AERO_E_INVALID, "aero_create: unknown storage format");
     if (!(grid->h > 0.0f) || !isfinite(grid->h)) return fail(nullptr, AERO_E_INVALID, "aero_create: bad cell size");
     if (!(sim->tau0 > 0.5f)) return fail(nullptr, AERO_E_INVALID, "aero_create: tau0 must exceed 0.5");
+    // Segment-major populations: 19 x 64 values per 64-cell row segment, addressed by the
+    // step kernel with 32-bit buffer offsets.
+    const uint64_t pop_bytes = (uint64_t)((grid->nx + 63) / 64) * grid->ny * grid->nz * 19 * 64 * store_bytes(storage);
+    if (pop_bytes > 0xffffffffull)
+        return fail(nullptr, AERO_E_INVALID, "aero_create: a population array must stay below 4 GiB");
 
     aero_ctx* c = new (std::nothrow) aero_ctx();
     if (!c) return fail(nullptr, AERO_E_NOMEM, "aero_create: host allocation failed");
@@ -127,12 +132,7 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) { free_ctx(c); set_global_error(std::string("hipGetDevice: ") + hipGetErrorString(e)); return AERO_E_HIP; }
     c->ns = (n + 255) & ~(uint64_t)255;
-    // Segment-major populations: 19 x 64 values per 64-cell row segment.
-    c->pop_bytes = (size_t)((grid->nx + 63) / 64) * grid->ny * grid->nz * 19 * 64 * store_bytes(storage);
-    if (c->pop_bytes > 0xffffffffull) {
-        delete c;
-        return fail(nullptr, AERO_E_INVALID, "aero_create: a population array must stay below 4 GiB");
-    }
+    c->pop_bytes = (size_t)pop_bytes;
     c->nslots = 0;
     for (int sh = 0; sh < kNumShapes; sh++) {
         uint32_t tx, ty, nt;

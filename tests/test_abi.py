@@ -67,6 +67,13 @@ def test_create_rejects_bad_arguments_before_any_gpu_call():
     bad_tau = _lib.Sim(0.05, 0.5, 0.1, 0.0)
     assert lib.aero_create(C.byref(g), C.byref(bad_tau), 0, C.byref(ctx)) == -1
     assert lib.aero_create(None, C.byref(sim), 0, C.byref(ctx)) == -1
+    # 1024^3 cells fit the cell limit, but one FP32 population array (16 segments per
+    # row x 1024 x 1024 x 19 x 64 x 4 B) would exceed the 4 GiB of a buffer offset;
+    # FP16 halves it and still exceeds it.
+    huge = _lib.Grid(1024, 1024, 1024, 1024 ** 3, 0, 0, 0, 0.1)
+    for storage in (0, 1):
+        assert lib.aero_create(C.byref(huge), C.byref(sim), storage, C.byref(ctx)) == -1
+        assert b"4 GiB" in lib.aero_global_error()
     assert lib.aero_step(None, 1) == -1
     assert lib.aero_last_error(None) == b"null context"
     with pytest.raises(aerolbm.AeroError, match="grid.n"):
