@@ -62,6 +62,7 @@ struct StepArgs {
     const uint8_t* flags;
     const uint32_t* fastmap;  // bit per 64-cell row segment: every cell plain interior fluid
     const uint32_t* edgemap;  // bit per segment: not fast, but no solid and no near-solid cell
+    const uint64_t* links;    // per cell: flag byte << 57 | solid pull sources by group (k_links)
     uint32_t nseg;            // 64-cell segments per row, ceil(nx / 64)
     float4* fields;         // written when write_fields != 0
     float* prev_u;          // outlet plane (x = nx-1) velocities, 3 floats per (y, z)
@@ -106,6 +107,7 @@ struct aero_ctx {
     float* emit;
     uint8_t* flags;
     uint32_t* fastmap;      // fast_words words of StepArgs::fastmap, then fast_words of edgemap
+    uint64_t* links;        // see StepArgs::links
     uint32_t nseg, fast_words;
     float4* fields;
     float* prev_u;

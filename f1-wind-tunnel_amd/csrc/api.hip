@@ -73,7 +73,7 @@ int upload_tau(aero_ctx* c) {
 void free_ctx(aero_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    void* ptrs[] = {c->f[0], c->f[1], c->solid, c->emit, c->flags, c->fastmap, c->fields, c->prev_u, c->temps[0],
+    void* ptrs[] = {c->f[0], c->f[1], c->solid, c->emit, c->flags, c->fastmap, c->links, c->fields, c->prev_u, c->temps[0],
                     c->temps[1], c->tau_x, c->slots, c->red_out, c->car_body, c->car_parts, c->d_rig,
                     c->parts, c->stream_pts, c->d_emitters};
     for (void* p : ptrs)
@@ -153,6 +153,7 @@ int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ct
         {(void**)&c->emit, n * 4},
         {(void**)&c->flags, c->ns},
         {(void**)&c->fastmap, (size_t)c->fast_words * 8},  // fast bits, then edge bits
+        {(void**)&c->links, n * 8},
         {(void**)&c->fields, n * 16},
         {(void**)&c->prev_u, outlet * 4},
         {(void**)&c->temps[0], n * 4},

@@ -208,7 +208,6 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
     const bool active = (x < NX) && (y < NY);
     // Cell indices stay 32-bit (fewer than 2^31 cells).
     const uint32_t idx = (uint32_t)x + (uint32_t)NX * ((uint32_t)y + (uint32_t)NY * (uint32_t)z);
-    const int32_t sxy = NX * NY;
     // Rows away from the floor, roof, symmetry plane and side ("inner" rows) whose
     // wavefront holds no solid and no solid-touching cell pull every population with a
     // plain shifted load (inlet/outlet lanes are patched afterwards).  Every other
@@ -233,7 +232,12 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
     }
     // Byte offset of population 0 of this lane's own cell; population I adds I * POPB.
     const uint32_t own = seg * SEGB + (uint32_t)lane * ESZ;
-    const uint32_t fl = (!fast && active) ? (uint32_t)*at(a.flags, idx) : 0u;
+    // A general wavefront reads each cell's link word: the flag byte (bits 57-63) and, per
+    // direction, whether the pull source is solid and of which group (bits 0-56), so its
+    // population loads wait for one memory round instead of the flag byte and then 18
+    // neighbour flag bytes.  An edge wavefront needs the flag byte alone.
+    const uint64_t lw = (!fast && !edge && active) ? a.links[idx] : 0ull;
+    const uint32_t fl = (!fast && active) ? (edge ? (uint32_t)*at(a.flags, idx) : (uint32_t)(lw >> 57)) : 0u;
     // tau0 + sponge ramps of this column (wgsl.ts:259-260), fetched with the populations
     // so that its latency is not paid again after the moments.
     const float tau_col = active ? *at(a.tau_x, (uint32_t)x) : 0.0f;
@@ -311,7 +315,6 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
             if (write_fields) a.fields[idx] = make_float4(0.f, 0.f, 0.f, 0.f);
             if (outlet) { a.prev_u[pidx] = 0.f; a.prev_u[pidx + 1] = 0.f; a.prev_u[pidx + 2] = 0.f; }
         } else {
-            const bool near = (fl & FL_NEAR) != 0;
             const bool duct = (fl & FL_DUCT) != 0;
             const bool floor0 = (fl & FL_FLOOR0) != 0;
             float pux = 0.f, puy = 0.f, puz = 0.f, puu15 = 0.f;
@@ -363,54 +366,21 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
                 // General path: the boundary rules of wgsl.ts:203-229 in the reference's
                 // order (inlet, outlet, moving floor, far field, symmetry mirror, solid
                 // bounce-back, interior), evaluated without serialising the loads:
-                //   (1) solid-touching lanes fetch their 18 neighbour groups together;
+                //   (1) the cell's link word (loaded with its flag byte) names its solid
+                //       pull sources and their groups;
                 //   (2) every direction gets a per-lane byte offset into the population
                 //       array (constant-valued pulls read the cell's own value);
                 //   (3) the 19 loads go out together, then constants, the floor term
                 //       and the momentum exchange are applied.
                 // Edge segments (no solid within one link) issued their loads above.
-                uint32_t s1 = 0u, s2 = 0u, s3 = 0u;  // bit I: source of direction I is solid of group 1/2/3
-                auto special = [&](auto ic) -> bool {  // inlet/outlet/floor/far field: no solid test
-                    constexpr int I = decltype(ic)::value;
-                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
-                    bool sp = false;
-                    if constexpr (cx == 1) sp = sp || (x == 0);
-                    if constexpr (cx == -1) sp = sp || outlet;
-                    if constexpr (cy == 1) sp = sp || flo_row;
-                    if constexpr (cy == -1) sp = sp || far_y;
-                    if constexpr (cz == -1) sp = sp || far_z;
-                    return sp;
-                };
-                auto src_off = [&](auto ic) -> int32_t {  // element shift of the source cell
-                    constexpr int I = decltype(ic)::value;
-                    constexpr int cx = kCX[I], cy = kCY[I], cz = kCZ[I];
-                    if constexpr (cz == 1) return mir_row ? cx + cy * NX : cx + cy * NX + sxy;
-                    else return cx + cy * NX + cz * sxy;
-                };
-                if (!edge && __builtin_amdgcn_ballot_w64(near) != 0) {
-                    uint32_t nb[19];
-                    for19([&](auto ic) {
-                        constexpr int I = decltype(ic)::value;
-                        if constexpr (I != 0) {
-                            const bool test = near && !special(ic);
-                            const uint32_t sidx = test ? (uint32_t)((int32_t)idx - src_off(ic)) : idx;
-                            nb[I] = *at(a.flags, sidx);
-                        }
-                    });
-                    for19([&](auto ic) {
-                        constexpr int I = decltype(ic)::value;
-                        if constexpr (I != 0) {
-                            const uint32_t g = (near && !special(ic)) ? (nb[I] & FL_GROUP) : 0u;
-                            s1 |= (g == 1u ? 1u : 0u) << I;
-                            s2 |= (g == 2u ? 1u : 0u) << I;
-                            s3 |= (g == 3u ? 1u : 0u) << I;
-                        }
-                    });
-                }
+                // bit I: the source of direction I is a solid of group 1 / 2 / 3 (a special
+                // pull, wgsl.ts:204-211, never tests its source)
+                const uint32_t s1 = (uint32_t)lw & 0x7ffffu, s2 = (uint32_t)(lw >> 19) & 0x7ffffu,
+                               s3 = (uint32_t)(lw >> 38) & 0x7ffffu;
                 const uint32_t sany = s1 | s2 | s3;
                 if (!edge) {
-                    // A solid source (never a special one: the ballot above tests only
-                    // non-special links) bounces back the cell's own opposite population.
+                    // A solid source (never a special one: k_links tests only non-special
+                    // links) bounces back the cell's own opposite population.
                     for19([&](auto ic) {
                         constexpr int I = decltype(ic)::value;
                         const uint32_t opp = own + (uint32_t)kOPP[I] * POPB;
@@ -667,7 +637,7 @@ __device__ __forceinline__ void step_tile(const StepArgs& a, const uint32_t l, c
 // XCD-contiguous tile order (SWZ), and a grid-stride persistent launch (PERSIST)
 // whose grid fills the chip once.  The autotuner picks among them per tunnel.
 template <int S, bool EXACT, int TX, bool SWZ>
-__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) k_step(StepArgs a) {
     static_assert(TX >= 64 && TX <= kBlock && kBlock % TX == 0, "whole wavefronts per row");
     step_tile<S, EXACT, TX>(a, SWZ ? xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x, a.fin, a.fout, a.write_fields);
 }
@@ -687,8 +657,8 @@ k_step_persist(StepArgs a) {
 // tile map with one scalar load (a search over the instances' tile offsets would be a
 // chain of dependent scalar loads ahead of every block's first population load) and
 // runs the same tile body with that instance's constants.  All instances share the
-// ping-pong parity.  Seven waves per SIMD (at most 72 VGPRs), as the single-tunnel
-// kernel reaches without a bound.
+// ping-pong parity.  Seven waves per SIMD (at most 72 VGPRs), as for the single-tunnel
+// kernel.
 template <int S, bool EXACT, int TX, bool SWZ>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) k_step_batch(BatchArgs b) {
     const uint32_t g = SWZ ? xcd_swizzle(blockIdx.x, gridDim.x) : blockIdx.x;
@@ -775,6 +745,33 @@ __global__ void __launch_bounds__(256) k_flags(const uint32_t* __restrict__ soli
     }
     if (hot) fl |= FL_HOTNB;
     flags[idx] = fl;
+}
+
+// Link word per cell (read by the step's general path): the flag byte in bits 57-63 and,
+// for a fluid cell next to a solid, bit i + 19 (g - 1) for every direction i whose pull
+// source is a solid of group g.  A source is tested as the step's solid rule tests it
+// (wgsl.ts:212-219): in range in x and y, below the far side in z, the symmetry plane
+// mirroring z < 0 to z = 0.
+__global__ void __launch_bounds__(256) k_links(const uint8_t* __restrict__ flags, uint64_t* __restrict__ links,
+                                              uint32_t nx, uint32_t ny, uint32_t nz) {
+    const uint64_t n = (uint64_t)nx * ny * nz;
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const int NX = nx, NY = ny, NZ = nz;
+    const int x = (int)(idx % nx), y = (int)((idx / nx) % ny), z = (int)(idx / ((uint64_t)nx * ny));
+    const uint32_t fl = flags[idx];
+    uint64_t w = (uint64_t)fl << 57;
+    if ((fl & FL_GROUP) == 0u && (fl & FL_NEAR) != 0u) {
+        for (int i = 1; i < 19; i++) {
+            const int sx = x - kCX[i], sy = y - kCY[i];
+            int sz = z - kCZ[i];
+            if (sx < 0 || sx >= NX || sy < 0 || sy >= NY || sz >= NZ) continue;
+            if (sz < 0) sz = 0;
+            const uint32_t g = flags[(uint64_t)sx + (uint64_t)NX * ((uint64_t)sy + (uint64_t)NY * sz)] & FL_GROUP;
+            if (g != 0u) w |= 1ull << (i + 19 * (int)(g - 1u));
+        }
+    }
+    links[idx] = w;
 }
 
 // Fixed-order reduction of the force slots: block b sums the contiguous chunk b of
@@ -892,6 +889,7 @@ void fill_step_consts(const aero_ctx* c, StepArgs* a) {
     }
     a->flags = c->flags;
     a->fastmap = c->fastmap;
+    a->links = c->links;
     a->edgemap = c->fastmap + c->fast_words;
     a->nseg = c->nseg;
     a->fields = c->fields;
@@ -918,6 +916,8 @@ hipError_t launch_flags(aero_ctx* c) {
     hipLaunchKernelGGL(k_flags, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->solid, c->emit,
                        c->flags, c->grid.nx, c->grid.ny, c->grid.nz);
     const uint64_t segs = (uint64_t)c->nseg * c->grid.ny * c->grid.nz;
+    hipLaunchKernelGGL(k_links, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, c->flags, c->links,
+                       c->grid.nx, c->grid.ny, c->grid.nz);
     hipLaunchKernelGGL(k_fastmap, dim3((unsigned)((segs + 255) / 256)), dim3(256), 0, c->stream, c->flags, c->fastmap,
                        c->fast_words, c->nseg, c->grid.nx, c->grid.ny, c->grid.nz);
     return hipGetLastError();
