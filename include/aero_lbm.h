@@ -105,7 +105,10 @@ const char* aero_last_error(const aero_ctx* ctx);
 
 /* Allocates the tunnel on the current HIP device (engine.ts:226-246).  Populations
  * are left uninitialised: call aero_init.  storage is AERO_STORE_*, optionally
- * OR-ed with AERO_MATH_EXACT. */
+ * OR-ed with AERO_MATH_EXACT.  Limits (AERO_E_INVALID, checked before any device
+ * call): at least 4x2x1 cells, at most 2^31 cells, nz <= 65535, tau0 > 0.5, and one
+ * population array below 4 GiB (ceil(nx/64) * 64 * ny * nz * 19 values of 4 bytes for
+ * FP32, 2 for FP16S / BF16S). */
 int aero_create(const aero_grid* grid, const aero_sim* sim, int storage, aero_ctx** out);
 int aero_destroy(aero_ctx* ctx);
 
