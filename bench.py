@@ -41,7 +41,7 @@ def bytes_per_update(storage: str) -> int:
     return 2 * 19 * s + 1
 
 
-def cpu_baseline(wl, budget_s: float = 15.0) -> dict:
+def cpu_baseline(wl, budget_s: float = 25.0) -> dict:
     """FP32 restatement of the reference kernels (OpenMP C), timed on a bounded
     sample of the same workload."""
     import numpy as np
